@@ -1,0 +1,66 @@
+"""Brute-force references for the DQMC numerics, from the mathematics only.
+
+B(t) = diag(exp(alpha*s_t)) @ expK with expK = exp(-dt*K), K = -nnt*A - mu*I,
+G(tau_t) = (1 + B(t)...B(1) B(L)...B(t+1))^{-1}.  Products and the inverse are done
+in mpmath at high precision on the FP64 input matrices, so the result is the exact
+inverse of those inputs; no stabilization is involved.
+"""
+import math
+
+import mpmath as mp
+import numpy as np
+
+
+def fourier_expk(nl, dt, nnt, mu, sign=-1.0):
+    """exp(sign*dt*K) from the plane-wave eigenbasis of the periodic square lattice:
+    eps_k = -2 nnt (cos kx + cos ky) - mu, counting periodic images like the adjacency does."""
+    n = nl * nl
+    ks = [2 * math.pi * m / nl for m in range(nl)]
+    out = np.zeros((n, n))
+    x = np.arange(n) % nl
+    y = np.arange(n) // nl
+    for kx in ks:
+        for ky in ks:
+            eps = -2.0 * nnt * (math.cos(kx) + math.cos(ky)) - mu
+            ph = np.exp(1j * (kx * x + ky * y))
+            out += (np.outer(ph, ph.conj()) * math.exp(sign * dt * eps)).real
+    return out / n
+
+
+def bmat(expk, alpha, s_t):
+    return np.exp(alpha * np.asarray(s_t, dtype=float))[:, None] * expk
+
+
+def _to_mp(a):
+    return mp.matrix(a.tolist())
+
+
+def _to_np(m):
+    return np.array(m.tolist(), dtype=float)
+
+
+def greens_mp(expk, alpha, fields, t=0, dps=120):
+    """(1 + B(t)...B(1) B(L)...B(t+1))^{-1} at `dps` digits; fields is [L][N]."""
+    L, n = fields.shape
+    with mp.workdps(dps):
+        E = _to_mp(expk)
+        M = mp.eye(n)
+        order = list(range(t + 1, L + 1)) + list(range(1, t + 1))   # rightmost first
+        for l in order:
+            d = [mp.e ** (mp.mpf(alpha) * int(fields[l % L][i])) for i in range(n)]
+            EM = E * M
+            for i in range(n):
+                for j in range(n):
+                    EM[i, j] *= d[i]
+            M = EM
+        G = (mp.eye(n) + M) ** -1
+        return _to_np(G)
+
+
+def greens_np(expk, alpha, fields, t=0):
+    """Plain FP64 version, adequate only for small beta."""
+    L, n = fields.shape
+    M = np.eye(n)
+    for l in list(range(t + 1, L + 1)) + list(range(1, t + 1)):
+        M = bmat(expk, alpha, fields[l % L]) @ M
+    return np.linalg.inv(np.eye(n) + M)

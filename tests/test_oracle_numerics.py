@@ -1,0 +1,173 @@
+"""Pins the oracle's numerics against mathematics (closed forms, brute force, invariants)."""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+
+import brute
+import oracle as orc
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_nn_hoppings_accumulate():
+    # square_lattice.cpp:48-61: nl=2 -> 2 on the single x and y neighbours, nl=1 -> 4 on the diagonal.
+    assert orc.nn_hoppings(1)[0, 0] == 4.0
+    a2 = orc.nn_hoppings(2)
+    assert a2[0, 1] == 2.0 and a2[0, 2] == 2.0 and a2[0, 3] == 0.0
+    a4 = orc.nn_hoppings(4)
+    assert np.allclose(a4, a4.T) and np.all(a4.sum(0) == 4)
+
+
+@pytest.mark.parametrize("nl,dt,nnt,mu", [(1, 0.1, 1.0, 0.0), (2, 0.1, 1.0, -0.5), (4, 0.05, 1.0, 0.0), (6, 0.1, 0.7, 0.3)])
+def test_expk_closed_form(nl, dt, nnt, mu):
+    e, ie = orc.expk(nl, dt, nnt, mu)
+    n = nl * nl
+    K = -nnt * orc.nn_hoppings(nl) - mu * np.eye(n)
+    assert np.abs(e - sla.expm(-dt * K)).max() < 1e-13
+    assert np.abs(ie - sla.expm(dt * K)).max() < 1e-13
+    assert np.abs(e - brute.fourier_expk(nl, dt, nnt, mu, -1.0)).max() < 1e-13
+    assert np.abs(e @ ie - np.eye(n)).max() < 1e-13
+
+
+def test_svd_reconstructs_graded_matrix():
+    rng = np.random.default_rng(0)
+    n = 24
+    q1, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    d = np.exp(np.linspace(30, -30, n))
+    a = (q1 * d) @ q2.T @ np.diag(np.exp(rng.uniform(-20, 20, n)))
+    rc, u, s, v = orc.svd(a)
+    assert rc == 0
+    assert np.all(np.diff(s) <= 0)
+    assert np.abs(u.T @ u - np.eye(n)).max() < 1e-13
+    assert np.abs(v.T @ v - np.eye(n)).max() < 1e-13
+    assert np.abs((u * s) @ v.T - a).max() / np.abs(a).max() < 1e-13
+    # singular values against numpy for a well-conditioned case
+    b = rng.standard_normal((n, n))
+    assert np.allclose(orc.svd(b)[2], np.linalg.svd(b, compute_uv=False), rtol=1e-12)
+
+
+def test_inverse_and_fullpiv_solve():
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((30, 30))
+    rc, ai = orc.inverse(a)
+    assert rc == 0 and np.abs(ai @ a - np.eye(30)).max() < 1e-12
+    b = rng.standard_normal((30, 30))
+    x = orc.fullpiv_qr_solve(a, b)
+    assert np.abs(a @ x - b).max() < 1e-11
+
+
+def _chain(nl, nt, beta, u, mu, pace, seed, eq=False, dyn=False):
+    c = orc.Chain(nl, nt, beta, u, mu, pace=pace, eqtime=eq, dynamic=dyn)
+    c.set_seed(seed)
+    c.fields_random()
+    return c
+
+
+@pytest.mark.parametrize("op", range(5))
+def test_bops_against_numpy(op):
+    c = _chain(3, 8, 1.0, 4.0, 0.2, 4, 7)
+    e, ie = c.expk()
+    rng = np.random.default_rng(op)
+    g = rng.standard_normal((9, 9))
+    t = 5
+    d = np.diag(np.exp(c.alpha * c.fields[t]))
+    di = np.diag(np.exp(-c.alpha * c.fields[t]))
+    want = [d @ e @ g, g @ d @ e, ie @ di @ g, g @ ie @ di, e @ d @ g][op]
+    assert np.abs(c.apply_bop(op, t, g) - want).max() < 1e-13
+
+
+@pytest.mark.parametrize("t", [0, 10, 20, 37, 40])
+def test_equaltime_gf_vs_mpmath(t):
+    # nl=4, beta=4, L=40, pace 10: general branch (0<t<L) and the two QR branches (t=0, t=L).
+    c = _chain(4, 40, 4.0, 4.0, -0.3, 10, 11)
+    rc, g = c.equaltime_gf_at(t)
+    assert rc == 0
+    e, _ = c.expk()
+    gm = brute.greens_mp(e, c.alpha, c.fields, t=t % 40, dps=80)
+    assert np.abs(g - gm).max() < 1e-10
+
+
+def test_init_gf_example_fields_vs_mpmath():
+    # The reference's own field fixture (example/ising.fields: nl=4, nt=160, beta=8, u=4, mu=0).
+    rows = np.loadtxt(os.path.join(GOLD, "example", "ising.fields"), skiprows=1, dtype=int)
+    f = np.zeros((160, 16), dtype=np.int32)
+    f[rows[:, 0], rows[:, 1]] = rows[:, 2]
+    c = orc.Chain(4, 160, 8.0, 4.0, 0.0, pace=10)
+    c.fields = f
+    assert c.init() == 0
+    e, _ = c.expk()
+    gm = brute.greens_mp(e, c.alpha, f, t=0, dps=140)
+    assert np.abs(c.gf(0) - gm).max() < 1e-10
+
+
+def test_ratio_and_sherman_morrison_invariant():
+    # One up sweep at tiny size: after the sweep G must equal (1+B_L..B_1)^{-1} of the
+    # final fields (so every accepted SM update and flip was consistent), and the
+    # acceptance probability min(1,r) matches the brute-force determinant ratio.
+    c = _chain(2, 8, 1.0, 4.0, 0.1, 4, 3)
+    assert c.init() == 0
+    e, _ = c.expk()
+    g0 = brute.greens_np(e, c.alpha, c.fields, 0)
+    assert np.abs(c.gf(0) - g0).max() < 1e-12
+    # determinant ratio for flipping (t=1, i=2) from G(1,1)
+    f = c.fields
+    g1 = brute.greens_np(e, c.alpha, f, 1)
+    a = c.alpha
+    delta = math.exp(-2 * a * f[1, 2]) - 1
+    r = (1 + (1 - g1[2, 2]) * delta) ** 2 / (1 + delta)
+    f2 = f.copy(); f2[1, 2] *= -1
+    def detw(ff):
+        M = np.eye(4)
+        for l in range(1, 9):
+            M = brute.bmat(e, a, ff[l % 8]) @ M
+        return np.linalg.det(np.eye(4) + M) ** 2 * math.exp(-a * ff.sum())
+    assert abs(r - detw(f2) / detw(f)) < 1e-12 * max(1, r)
+    assert c.sweep_up() == 0
+    gf = brute.greens_np(e, c.alpha, c.fields, 0)
+    assert np.abs(c.gf(0) - gf).max() < 1e-11
+    st = c.stats()
+    assert st["visits"] == 8 * 4 and 0 < st["accepts"] <= st["visits"]
+
+
+def test_u0_closed_form():
+    # U=0: alpha=0, every move accepted, G = (1+exp(-beta K))^{-1} exactly.
+    c = _chain(4, 20, 2.0, 0.0, -0.4, 5, 5, eq=True)
+    c.set_thermalization(False)
+    assert c.init() == 0 and c.alpha == 0.0
+    c.sweep_up()
+    c.sweep_down()
+    st = c.stats()
+    assert st["accepts"] == st["visits"]
+    K = -orc.nn_hoppings(4) + 0.4 * np.eye(16)
+    g = np.linalg.inv(np.eye(16) + sla.expm(-2.0 * K))
+    assert np.abs(c.gf(0) - g).max() < 1e-12
+    for t in (0, 7, 19):
+        assert np.abs(c.gf(1, t) - g).max() < 1e-12
+
+
+def test_atomic_limit_double_occupancy():
+    # nnt=0: K = -mu I is diagonal; sites decouple and the Trotter error vanishes.
+    # <n_up n_dn> per site = e^{beta U/4} / (2 e^{beta U/4} + 2 e^{-beta U/4}) at mu=0 (attractive U).
+    nl, nt, beta, u = 2, 20, 2.0, 2.0
+    c = orc.Chain(nl, nt, beta, u, 0.0, nnt=0.0, pace=5, eqtime=True)
+    c.set_seed(42)
+    c.fields_random()
+    c.init()
+    c.set_thermalization(True)
+    for _ in range(20):
+        c.sweep_up(); c.sweep_down()
+    c.set_thermalization(False)
+    acc = []
+    for _ in range(400):
+        c.sweep_up()
+        acc.append(c.measure_equaltime()[1])
+        c.sweep_down()
+        acc.append(c.measure_equaltime()[1])
+    acc = np.array(acc)
+    want = math.exp(beta * u / 4) / (2 * math.exp(beta * u / 4) + 2 * math.exp(-beta * u / 4))
+    err = acc.std() / math.sqrt(len(acc) / 20)   # generous autocorrelation allowance
+    assert abs(acc.mean() - want) < 5 * err + 1e-3
