@@ -1,0 +1,928 @@
+// dqmc_engine.hip -- host side of the MI355X DQMC sweep engine and its C ABI.
+//
+// The sweep schedule mirrors DQMC::Core (src/dqmc_core.cpp): the same slice order, the same
+// stabilization points (t % pace == 0 || t == nt going up, (t-1) % pace == 0 going down),
+// the same left/right stack roles and the same wrap-error bookkeeping.  What differs is how
+// each step is computed:
+//   * all chains of a context advance together; every step is one batched kernel launch;
+//   * B(t) = diag(e^{dt mu + alpha s_t}) exp(dt nnt A) is applied through fused diagonal
+//     scalings of one shared exp(dt nnt A) (the chemical potential commutes with the hopping
+//     matrix, so a per-chain mu costs nothing);
+//   * the UDV stacks are built by Householder QR on columns pre-sorted by norm instead of
+//     SVD, and the stacks keep the accumulated V (no V_0 V_1 ... recomputation);
+//   * (1 + L R^T)^{-1} is recomposed by solving with a QR factorization instead of an
+//     explicit LU inverse.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/dqmc_amd.h"
+#include "dqmc_kernels.h"
+
+using namespace dqmc;
+
+namespace {
+
+struct DStack {
+    double* U = nullptr;  // [B][S][N*N]
+    double* V = nullptr;  // [B][S][N*N]  accumulated V
+    double* D = nullptr;  // [B][S][N]
+    int len = 0;
+};
+
+const uint64_t kMinstdM = 2147483647ULL;
+
+double host_canonical(uint64_t& x)
+{
+    x = (x * 16807ULL) % kMinstdM;
+    const uint64_t x1 = x;
+    x = (x * 16807ULL) % kMinstdM;
+    const uint64_t x2 = x;
+    const double R = 2147483646.0;
+    const double R2 = 4611686009837453312.0;
+    double u = std::fma((double)(x2 - 1ULL), R, (double)(x1 - 1ULL)) / R2;
+    if (u >= 1.0) u = std::nextafter(1.0, 0.0);
+    return u;
+}
+
+}  // namespace
+
+struct dqmc_ctx {
+    // parameters
+    int nl = 0, N = 0, L = 0, pace = 0, S = 0, B = 0, prec = DQMC_FP64, kmax = 1;
+    double dt = 0, beta = 0, nnt = 0, u = 0, mu = 0, alpha = 0;
+    bool eqtime = false, dynamic = false, thermalization = false, use_graphs = false;
+    bool initialized = false;
+    int current_t = 0;
+    long long updating_sweeps = 0;
+    std::vector<double> chain_mu;
+    std::vector<uint64_t> rng_host;
+    std::string err;
+    int tile64 = TILE_64x64, tile32 = TILE_64x64;
+
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+    std::map<int, hipGraphExec_t> graphs;
+
+    // device memory
+    std::vector<void*> allocs;
+    double *expK = nullptr, *invExpK = nullptr;
+    float *expK32 = nullptr, *invExpK32 = nullptr;
+    double *tab = nullptr, *itab = nullptr, *delta = nullptr;
+    uint64_t *powtab = nullptr, *rng = nullptr;
+    unsigned long long* acc_count = nullptr;
+    uint32_t* accmask = nullptr;
+    int8_t* fields = nullptr;
+    double *G64 = nullptr;
+    float *G32 = nullptr, *T32 = nullptr;
+    double *stkA = nullptr, *stkB = nullptr, *stk = nullptr, *stk_alt = nullptr;
+    double *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *qr = nullptr;
+    int* perm = nullptr;
+    double *gt0A = nullptr, *gt0B = nullptr, *g0tA = nullptr, *g0tB = nullptr, *gt0 = nullptr, *g0t = nullptr;
+    double *vecgftt = nullptr, *vecgft0 = nullptr, *vecgf0t = nullptr;
+    double *wrap_eq = nullptr, *wrap_dyn = nullptr, *maxdG = nullptr;
+    DStack left, right;
+
+    long long NN() const { return (long long)N * N; }
+    long long sqr() const { return (long long)N * 3 * N; }
+
+    int fail(int code, const char* fmt, ...)
+    {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+    template <typename P>
+    int alloc(P** p, size_t bytes)
+    {
+        void* q = nullptr;
+        if (hipMalloc(&q, bytes < 16 ? 16 : bytes) != hipSuccess) return fail(DQMC_ENOMEM, "hipMalloc(%zu) failed", bytes);
+        allocs.push_back(q);
+        *p = (P*)q;
+        return 0;
+    }
+};
+
+#define HCHECK(c, expr)                                                                            \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess) return (c)->fail(DQMC_EHIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                                               __FILE__, __LINE__);                                \
+    } while (0)
+#define RCHECK(expr)                \
+    do {                            \
+        int _r = (expr);            \
+        if (_r) return _r;          \
+    } while (0)
+
+namespace {
+
+// exp(s * dt * nnt * A) for the periodic nl x nl square lattice, from its plane-wave
+// eigenbasis: the adjacency (src/square_lattice.cpp:48-61) has eigenvalues
+// 2 cos kx + 2 cos ky with multiplicities counted like the accumulated entries.
+void lattice_expk(int nl, double x, std::vector<double>& out)
+{
+    const int N = nl * nl;
+    std::vector<double> f(N, 0.0);  // value for displacement (dx, dy)
+    for (int dy = 0; dy < nl; ++dy)
+        for (int dx = 0; dx < nl; ++dx) {
+            double s = 0.0;
+            for (int my = 0; my < nl; ++my)
+                for (int mx = 0; mx < nl; ++mx) {
+                    const double kx = 2.0 * M_PI * mx / nl, ky = 2.0 * M_PI * my / nl;
+                    s += std::cos(kx * dx + ky * dy) * std::exp(x * 2.0 * (std::cos(kx) + std::cos(ky)));
+                }
+            f[dx + nl * dy] = s / N;
+        }
+    out.assign((size_t)N * N, 0.0);
+    for (int j = 0; j < N; ++j)
+        for (int i = 0; i < N; ++i) {
+            const int dx = ((j % nl) - (i % nl) + nl) % nl, dy = ((j / nl) - (i / nl) + nl) % nl;
+            out[i + (size_t)j * N] = f[dx + nl * dy];
+        }
+}
+
+DiagArg dnone() { return DiagArg(); }
+DiagArg dvec(int mode, const double* v, long long vstride)
+{
+    DiagArg d;
+    d.mode = mode; d.v = v; d.vstride = vstride;
+    return d;
+}
+DiagArg dfield(dqmc_ctx* c, const double* table, int t)
+{
+    DiagArg d;
+    d.mode = DIAG_FIELD; d.v = table; d.vstride = 2;
+    d.f = c->fields + (long long)t * c->N; d.fstride = (long long)c->L * c->N;
+    return d;
+}
+
+template <typename T>
+int gemm(dqmc_ctx* c, const T* A, long long sA, int lda, int tA, const T* B, long long sB, int ldb, int tB, T* C,
+         long long sC, int ldc, int tC, DiagArg rowd = DiagArg(), DiagArg innd = DiagArg(),
+         DiagArg cold = DiagArg(), double alpha = 1.0, double beta = 0.0)
+{
+    GemmArgs<T> g;
+    g.M = c->N; g.Nc = c->N; g.K = c->N;
+    g.A = A; g.sA = sA; g.lda = lda; g.tA = tA;
+    g.B = B; g.sB = sB; g.ldb = ldb; g.tB = tB;
+    g.C = C; g.sC = sC; g.ldc = ldc; g.tC = tC;
+    g.alpha = alpha; g.beta = beta;
+    g.rowd = rowd; g.innd = innd; g.cold = cold;
+    g.batch = c->B;
+    HCHECK(c, launch_gemm<T>(g, sizeof(T) == 8 ? c->tile64 : c->tile32, c->stream));
+    return 0;
+}
+
+// ---- slice operations --------------------------------------------------------------------
+// G <- B(t) G B(t)^{-1}  (Core::wrap_from_0_to_beta, src/dqmc_core.cpp:142-147)
+int wrap_up(dqmc_ctx* c, int t)
+{
+    const long long nn = c->NN();
+    if (c->prec == DQMC_MIXED) {
+        RCHECK(gemm<float>(c, c->expK32, 0, c->N, 0, c->G32, nn, c->N, 0, c->T32, nn, c->N, 0));
+        RCHECK(gemm<float>(c, c->T32, nn, c->N, 0, c->invExpK32, 0, c->N, 0, c->G32, nn, c->N, 0,
+                           dfield(c, c->tab, t), dnone(), dfield(c, c->itab, t)));
+    } else {
+        RCHECK(gemm<double>(c, c->expK, 0, c->N, 0, c->G64, nn, c->N, 0, c->W1, nn, c->N, 0));
+        RCHECK(gemm<double>(c, c->W1, nn, c->N, 0, c->invExpK, 0, c->N, 0, c->G64, nn, c->N, 0,
+                            dfield(c, c->tab, t), dnone(), dfield(c, c->itab, t)));
+    }
+    return 0;
+}
+// G <- B(t)^{-1} G B(t)  (Core::wrap_from_beta_to_0, src/dqmc_core.cpp:156-161)
+int wrap_down(dqmc_ctx* c, int t)
+{
+    const long long nn = c->NN();
+    if (c->prec == DQMC_MIXED) {
+        RCHECK(gemm<float>(c, c->G32, nn, c->N, 0, c->expK32, 0, c->N, 0, c->T32, nn, c->N, 0, dnone(),
+                           dfield(c, c->tab, t)));
+        RCHECK(gemm<float>(c, c->invExpK32, 0, c->N, 0, c->T32, nn, c->N, 0, c->G32, nn, c->N, 0, dnone(),
+                           dfield(c, c->itab, t)));
+    } else {
+        RCHECK(gemm<double>(c, c->G64, nn, c->N, 0, c->expK, 0, c->N, 0, c->W1, nn, c->N, 0, dnone(),
+                            dfield(c, c->tab, t)));
+        RCHECK(gemm<double>(c, c->invExpK, 0, c->N, 0, c->W1, nn, c->N, 0, c->G64, nn, c->N, 0, dnone(),
+                            dfield(c, c->itab, t)));
+    }
+    return 0;
+}
+// stacks <- B(t) stacks (src/dqmc_core.cpp:190) or B(t)^T stacks (:251, adjB, src/hubbard.cpp:333-341)
+int chain_mul(dqmc_ctx* c, int t, bool adjoint)
+{
+    const long long nn = c->NN();
+    if (!adjoint)
+        RCHECK(gemm<double>(c, c->expK, 0, c->N, 0, c->stk, nn, c->N, 0, c->stk_alt, nn, c->N, 0,
+                            dfield(c, c->tab, t)));
+    else
+        RCHECK(gemm<double>(c, c->expK, 0, c->N, 0, c->stk, nn, c->N, 0, c->stk_alt, nn, c->N, 0, dnone(),
+                            dfield(c, c->tab, t)));
+    std::swap(c->stk, c->stk_alt);
+    return 0;
+}
+int metropolis(dqmc_ctx* c, int t)
+{
+    if (c->prec == DQMC_MIXED) {
+        MetroArgs<float> a;
+        a.G = c->G32; a.sG = c->NN();
+        a.fields = c->fields; a.N = c->N; a.L = c->L; a.t = t;
+        a.rng = c->rng; a.powtab = c->powtab; a.delta = c->delta;
+        a.accmask = c->accmask; a.acc_count = c->acc_count; a.kmax = c->kmax;
+        HCHECK(c, launch_metropolis<float>(a, c->B, c->stream));
+    } else {
+        MetroArgs<double> a;
+        a.G = c->G64; a.sG = c->NN();
+        a.fields = c->fields; a.N = c->N; a.L = c->L; a.t = t;
+        a.rng = c->rng; a.powtab = c->powtab; a.delta = c->delta;
+        a.accmask = c->accmask; a.acc_count = c->acc_count; a.kmax = c->kmax;
+        HCHECK(c, launch_metropolis<double>(a, c->B, c->stream));
+    }
+    return 0;
+}
+
+// ---- UDV stack push (svd_stack.hpp:107-175 restated with QR) ------------------------------
+double* sU(dqmc_ctx* c, DStack& s, int k) { return s.U + (long long)k * c->NN(); }
+double* sV(dqmc_ctx* c, DStack& s, int k) { return s.V + (long long)k * c->NN(); }
+double* sD(dqmc_ctx* c, DStack& s, int k) { return s.D + (long long)k * c->N; }
+
+int stack_push(dqmc_ctx* c, DStack& s)
+{
+    const long long nn = c->NN(), st = (long long)c->S * nn, sd = (long long)c->S * c->N;
+    const int N = c->N;
+    if (s.len >= c->S) return c->fail(DQMC_ESTATE, "stack overflow");
+    const double* M = c->stk;
+    if (s.len > 0) {
+        // M = (stacks * U_top) * diag(D_top)   (svd_stack.hpp:144)
+        RCHECK(gemm<double>(c, c->stk, nn, N, 0, sU(c, s, s.len - 1), st, N, 0, c->W1, nn, N, 0, dnone(), dnone(),
+                            dvec(DIAG_VEC, sD(c, s, s.len - 1), sd)));
+        M = c->W1;
+    }
+    HCHECK(c, launch_udv_prepare(M, nn, c->qr, c->sqr(), c->perm, N, c->B, c->stream));
+    HCHECK(c, launch_qr(c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
+    HCHECK(c, launch_udv_finish(c->qr, c->sqr(), c->perm, sU(c, s, s.len), st, sD(c, s, s.len), sd, c->W2, nn, N,
+                                c->B, c->stream));
+    if (s.len == 0)
+        HCHECK(c, launch_copy_d(c->W2, nn, sV(c, s, 0), st, nn, c->B, c->stream));
+    else
+        RCHECK(gemm<double>(c, sV(c, s, s.len - 1), st, N, 0, c->W2, nn, N, 0, sV(c, s, s.len), st, N, 0));
+    s.len += 1;
+    return 0;
+}
+
+// (1 + u diag(d) v^T)^{-1} with u orthogonal (compute_gf_00bb, stable_numerics.hpp:136-155);
+// result written to out, transposed when `transposed_result` (left-stack-empty case).
+int solve_00bb(dqmc_ctx* c, const double* u, const double* v, const double* d, double* out, bool transposed_result)
+{
+    const long long nn = c->NN(), st = (long long)c->S * nn, sd = (long long)c->S * c->N;
+    const int N = c->N;
+    HCHECK(c, launch_build_00bb(u, st, v, st, d, sd, c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
+    HCHECK(c, launch_qr(c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
+    HCHECK(c, launch_trsm_upper(c->qr, c->sqr(), N, 2 * N, N, N, c->B, c->stream));
+    HCHECK(c, launch_extract(c->qr, c->sqr(), 2 * N, N, out, nn, N, transposed_result ? 0 : 1, c->B, c->stream));
+    return 0;
+}
+
+// Equal-time G from the stack tops into `out` (compute_equaltime_gf, stable_numerics.hpp:186-239).
+int recompute_eq(dqmc_ctx* c, double* out)
+{
+    const long long nn = c->NN(), st = (long long)c->S * nn, sd = (long long)c->S * c->N;
+    const int N = c->N, W = 2 * N;
+    DStack &Ls = c->left, &Rs = c->right;
+    if (Ls.len == 0) {
+        // G^T = (1 + Ur Dr Vr^T)^{-1}
+        return solve_00bb(c, sU(c, Rs, Rs.len - 1), sV(c, Rs, Rs.len - 1), sD(c, Rs, Rs.len - 1), out, true);
+    }
+    if (Rs.len == 0) return solve_00bb(c, sU(c, Ls, Ls.len - 1), sV(c, Ls, Ls.len - 1), sD(c, Ls, Ls.len - 1), out, false);
+    const double *Ul = sU(c, Ls, Ls.len - 1), *Vl = sV(c, Ls, Ls.len - 1), *Dl = sD(c, Ls, Ls.len - 1);
+    const double *Ur = sU(c, Rs, Rs.len - 1), *Vr = sV(c, Rs, Rs.len - 1), *Dr = sD(c, Rs, Rs.len - 1);
+    // temp = Dlmax^{-1} (Ul^T Ur) Drmax^{-1} + Dlmin (Vl^T Vr) Drmin, written row-major into qr
+    RCHECK(gemm<double>(c, Ul, st, N, 1, Ur, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_INVMAX1, Dl, sd), dnone(),
+                        dvec(DIAG_INVMAX1, Dr, sd)));
+    RCHECK(gemm<double>(c, Vl, st, N, 1, Vr, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_MIN1, Dl, sd), dnone(),
+                        dvec(DIAG_MIN1, Dr, sd), 1.0, 1.0));
+    HCHECK(c, launch_build_rhs(Ul, st, dvec(DIAG_INVMAX1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, N, c->B, c->stream));
+    HCHECK(c, launch_qr(c->qr, c->sqr(), N, W, c->B, c->stream));
+    HCHECK(c, launch_trsm_upper(c->qr, c->sqr(), N, W, N, N, c->B, c->stream));
+    // G = Ur Drmax^{-1} X
+    RCHECK(gemm<double>(c, Ur, st, N, 0, c->qr + N, c->sqr(), W, 1, out, nn, N, 0, dnone(), dvec(DIAG_INVMAX1, Dr, sd)));
+    return 0;
+}
+
+// Dynamic recompute (compute_dynamic_gf, stable_numerics.hpp:245-324) into gtt, gt0, g0t.
+int recompute_dyn(dqmc_ctx* c, double* gtt, double* gt0n, double* g0tn)
+{
+    const long long nn = c->NN(), st = (long long)c->S * nn, sd = (long long)c->S * c->N;
+    const int N = c->N;
+    DStack &Ls = c->left, &Rs = c->right;
+    if (Ls.len == 0) {
+        RCHECK(solve_00bb(c, sU(c, Rs, Rs.len - 1), sV(c, Rs, Rs.len - 1), sD(c, Rs, Rs.len - 1), gtt, true));
+        HCHECK(c, launch_copy_d(gtt, nn, gt0n, nn, nn, c->B, c->stream));
+        HCHECK(c, launch_axpi(gtt, nn, g0tn, nn, 1.0, -1.0, N, c->B, c->stream));
+        return 0;
+    }
+    if (Rs.len == 0) {
+        RCHECK(solve_00bb(c, sU(c, Ls, Ls.len - 1), sV(c, Ls, Ls.len - 1), sD(c, Ls, Ls.len - 1), gtt, false));
+        HCHECK(c, launch_axpi(gtt, nn, gt0n, nn, -1.0, 1.0, N, c->B, c->stream));
+        HCHECK(c, launch_axpi(gtt, nn, g0tn, nn, -1.0, 0.0, N, c->B, c->stream));
+        return 0;
+    }
+    const double *Ul = sU(c, Ls, Ls.len - 1), *Vl = sV(c, Ls, Ls.len - 1), *Dl = sD(c, Ls, Ls.len - 1);
+    const double *Ur = sU(c, Rs, Rs.len - 1), *Vr = sV(c, Rs, Rs.len - 1), *Dr = sD(c, Rs, Rs.len - 1);
+    const int W = 3 * N;
+    RCHECK(gemm<double>(c, Ul, st, N, 1, Ur, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_INVMAX1, Dl, sd), dnone(),
+                        dvec(DIAG_INVMAX1, Dr, sd)));
+    RCHECK(gemm<double>(c, Vl, st, N, 1, Vr, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_MIN1, Dl, sd), dnone(),
+                        dvec(DIAG_MIN1, Dr, sd), 1.0, 1.0));
+    HCHECK(c, launch_build_rhs(Ul, st, dvec(DIAG_INVMAX1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, N, c->B, c->stream));
+    HCHECK(c, launch_build_rhs(Vl, st, dvec(DIAG_MIN1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, 2 * N, c->B, c->stream));
+    HCHECK(c, launch_qr(c->qr, c->sqr(), N, W, c->B, c->stream));
+    HCHECK(c, launch_trsm_upper(c->qr, c->sqr(), N, W, N, 2 * N, c->B, c->stream));
+    RCHECK(gemm<double>(c, Ur, st, N, 0, c->qr + N, c->sqr(), W, 1, gtt, nn, N, 0, dnone(), dvec(DIAG_INVMAX1, Dr, sd)));
+    RCHECK(gemm<double>(c, Ur, st, N, 0, c->qr + 2 * N, c->sqr(), W, 1, gt0n, nn, N, 0, dnone(),
+                        dvec(DIAG_INVMAX1, Dr, sd)));
+    // G(0,t) = -B(beta,t) G(t,t) = -(Vr Dr Ur^T) Ur Drmax^{-1} X1 = -Vr Drmin X1, with
+    // X1 = temp^{-1} Dlmax^{-1} Ul^T from the solve above.  This needs only Ur orthogonal;
+    // the reference's form (stable_numerics.hpp:305-323) also assumes an orthogonal Vl, which
+    // holds for its SVD stacks but not for QR-based ones.
+    RCHECK(gemm<double>(c, Vr, st, N, 0, c->qr + N, c->sqr(), W, 1, g0tn, nn, N, 0, dnone(), dvec(DIAG_MIN1, Dr, sd),
+                        dnone(), -1.0));
+    return 0;
+}
+
+int set_identity_stk(dqmc_ctx* c)
+{
+    c->stk = c->stkA;
+    c->stk_alt = c->stkB;
+    HCHECK(c, launch_set_identity(c->stk, c->NN(), c->N, 1.0, c->B, c->stream));
+    return 0;
+}
+
+int store_current(dqmc_ctx* c, double* vec, int t)
+{
+    const long long nn = c->NN(), sv = (long long)c->L * nn;
+    if (c->prec == DQMC_MIXED)
+        HCHECK(c, launch_copy_f2d(c->G32, nn, vec + (long long)t * nn, sv, nn, c->B, c->stream));
+    else
+        HCHECK(c, launch_copy_d(c->G64, nn, vec + (long long)t * nn, sv, nn, c->B, c->stream));
+    return 0;
+}
+
+int stabilize_eq(dqmc_ctx* c)
+{
+    RCHECK(recompute_eq(c, c->W3));
+    HCHECK(c, launch_compare_commit(c->W3, c->G64, c->prec == DQMC_MIXED ? c->G32 : nullptr, c->NN(), c->N, c->wrap_eq,
+                                    c->maxdG, c->B, c->stream));
+    return set_identity_stk(c);
+}
+
+// ---- sweeps ---------------------------------------------------------------------------------
+int do_sweep_up(dqmc_ctx* c)
+{
+    const int L = c->L;
+    const bool store = !c->thermalization && c->eqtime;
+    RCHECK(set_identity_stk(c));
+    for (int t = 1; t <= L; ++t) {
+        RCHECK(wrap_up(c, t % L));
+        RCHECK(metropolis(c, t % L));
+        RCHECK(chain_mul(c, t % L, false));
+        if (t % c->pace == 0 || t == L) {
+            c->right.len -= 1;
+            RCHECK(stack_push(c, c->left));
+            RCHECK(stabilize_eq(c));
+        }
+        if (store) RCHECK(store_current(c, c->vecgftt, t % L));
+    }
+    return 0;
+}
+
+int do_sweep_down(dqmc_ctx* c)
+{
+    const int L = c->L;
+    const bool store = !c->thermalization && c->eqtime;
+    RCHECK(set_identity_stk(c));
+    for (int t = L; t >= 1; --t) {
+        RCHECK(metropolis(c, t % L));
+        RCHECK(chain_mul(c, t % L, true));
+        RCHECK(wrap_down(c, t % L));
+        if ((t - 1) % c->pace == 0) {
+            c->left.len -= 1;
+            RCHECK(stack_push(c, c->right));
+            RCHECK(stabilize_eq(c));
+        }
+        if (store) RCHECK(store_current(c, c->vecgftt, (t - 1) % L));
+    }
+    return 0;
+}
+
+int do_sweep_dynamic(dqmc_ctx* c)
+{
+    const int L = c->L, N = c->N;
+    const long long nn = c->NN(), sv = (long long)L * nn;
+    c->gt0 = c->gt0A; c->g0t = c->g0tA;
+    if (c->prec == DQMC_MIXED) HCHECK(c, launch_copy_f2d(c->G32, nn, c->W1, nn, nn, c->B, c->stream));
+    else HCHECK(c, launch_copy_d(c->G64, nn, c->W1, nn, nn, c->B, c->stream));
+    HCHECK(c, launch_copy_d(c->W1, nn, c->gt0, nn, nn, c->B, c->stream));
+    HCHECK(c, launch_axpi(c->W1, nn, c->g0t, nn, 1.0, -1.0, N, c->B, c->stream));
+    RCHECK(set_identity_stk(c));
+    for (int t = 1; t <= L; ++t) {
+        const int ti = t % L;
+        RCHECK(wrap_up(c, ti));
+        double* gt0n = (c->gt0 == c->gt0A) ? c->gt0B : c->gt0A;
+        double* g0tn = (c->g0t == c->g0tA) ? c->g0tB : c->g0tA;
+        RCHECK(gemm<double>(c, c->expK, 0, N, 0, c->gt0, nn, N, 0, gt0n, nn, N, 0, dfield(c, c->tab, ti)));
+        RCHECK(gemm<double>(c, c->g0t, nn, N, 0, c->invExpK, 0, N, 0, g0tn, nn, N, 0, dnone(), dnone(),
+                            dfield(c, c->itab, ti)));
+        c->gt0 = gt0n; c->g0t = g0tn;
+        RCHECK(chain_mul(c, ti, false));
+        if (t % c->pace == 0 || t == L) {
+            c->right.len -= 1;
+            RCHECK(stack_push(c, c->left));
+            RCHECK(recompute_dyn(c, c->W3, c->W2, c->W1));
+            HCHECK(c, launch_compare_commit(c->W3, c->G64, c->prec == DQMC_MIXED ? c->G32 : nullptr, nn, N, c->wrap_eq,
+                                            c->maxdG, c->B, c->stream));
+            HCHECK(c, launch_compare_only(c->W2, c->gt0, nn, N, c->wrap_dyn, c->B, c->stream));
+            HCHECK(c, launch_compare_only(c->W1, c->g0t, nn, N, c->wrap_dyn, c->B, c->stream));
+            HCHECK(c, launch_copy_d(c->W2, nn, c->gt0, nn, nn, c->B, c->stream));
+            HCHECK(c, launch_copy_d(c->W1, nn, c->g0t, nn, nn, c->B, c->stream));
+            RCHECK(set_identity_stk(c));
+        }
+        RCHECK(store_current(c, c->vecgftt, ti));
+        if (ti != 0) {
+            HCHECK(c, launch_copy_d(c->gt0, nn, c->vecgft0 + (long long)ti * nn, sv, nn, c->B, c->stream));
+            HCHECK(c, launch_copy_d(c->g0t, nn, c->vecgf0t + (long long)ti * nn, sv, nn, c->B, c->stream));
+        } else {
+            RCHECK(store_current(c, c->vecgft0, 0));
+            HCHECK(c, launch_axpi(c->vecgft0, sv, c->vecgf0t, sv, 1.0, -1.0, N, c->B, c->stream));
+        }
+    }
+    return 0;
+}
+
+// Run a sweep eagerly, or through a captured graph keyed by its kind and store flag.
+typedef int (*sweep_fn)(dqmc_ctx*);
+int run_sweep(dqmc_ctx* c, sweep_fn fn, int kind)
+{
+    if (!c->use_graphs) {
+        RCHECK(fn(c));
+        HCHECK(c, hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    const int key = kind * 2 + ((!c->thermalization && c->eqtime) ? 1 : 0);
+    auto it = c->graphs.find(key);
+    const int lenL = c->left.len, lenR = c->right.len;
+    if (it == c->graphs.end()) {
+        hipGraph_t g;
+        HCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int rc = fn(c);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (rc) return rc;
+        HCHECK(c, e);
+        hipGraphExec_t ex;
+        HCHECK(c, hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+        HCHECK(c, hipGraphDestroy(g));
+        c->graphs[key] = ex;
+        it = c->graphs.find(key);
+        // the capture only recorded the launches; restore the host-side state and replay
+        c->left.len = lenL; c->right.len = lenR;
+    }
+    HCHECK(c, hipGraphLaunch(it->second, c->stream));
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    // host-side bookkeeping of the replayed sweep
+    if (kind == 1) { c->right.len = c->S; c->left.len = 0; }
+    else { c->left.len = c->S; c->right.len = 0; }
+    return 0;
+}
+
+int autotune(dqmc_ctx* c)
+{
+    // Pick the GEMM tile per precision by timing the wrap GEMM shape on this context's batch.
+    hipEvent_t e0, e1;
+    HCHECK(c, hipEventCreate(&e0));
+    HCHECK(c, hipEventCreate(&e1));
+    const long long nn = c->NN();
+    HCHECK(c, launch_set_identity(c->W1, nn, c->N, 1.0, c->B, c->stream));
+    if (c->prec == DQMC_MIXED) HCHECK(c, launch_copy_d2f(c->W1, nn, c->T32, nn, nn, c->B, c->stream));
+    for (int p = 0; p < 2; ++p) {
+        if (p == 1 && c->prec != DQMC_MIXED) break;
+        float best = 1e30f;
+        int bt = TILE_64x64;
+        for (int tile = 0; tile < TILE_COUNT; ++tile) {
+            if (p == 0) c->tile64 = tile; else c->tile32 = tile;
+            for (int rep = 0; rep < 2; ++rep) {
+                HCHECK(c, hipEventRecord(e0, c->stream));
+                for (int it = 0; it < 3; ++it) {
+                    if (p == 0) RCHECK(gemm<double>(c, c->expK, 0, c->N, 0, c->W1, nn, c->N, 0, c->W2, nn, c->N, 0));
+                    else RCHECK(gemm<float>(c, c->expK32, 0, c->N, 0, c->T32, nn, c->N, 0, (float*)c->W2, nn, c->N, 0));
+                }
+                HCHECK(c, hipEventRecord(e1, c->stream));
+                HCHECK(c, hipEventSynchronize(e1));
+                float ms = 0;
+                HCHECK(c, hipEventElapsedTime(&ms, e0, e1));
+                if (rep == 1 && ms < best) { best = ms; bt = tile; }
+            }
+        }
+        if (p == 0) c->tile64 = bt; else c->tile32 = bt;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return 0;
+}
+
+}  // namespace
+
+// ================================== C ABI ====================================================
+extern "C" {
+
+int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k, dqmc_ctx** out)
+{
+    if (!p || !out || n_chains <= 0) return DQMC_EINVAL;
+    *out = nullptr;
+    if (p->nl < 1 || p->nt < 1 || p->stabilization_pace < 1 || p->u < 0.0 || p->beta < 0.0) return DQMC_EINVAL;
+    if (precision != DQMC_FP64 && precision != DQMC_MIXED) return DQMC_EINVAL;
+    const int N = p->nl * p->nl;
+    if (N > 256) return DQMC_EINVAL;
+    if (delay_k < 1 || delay_k > 64) return DQMC_EINVAL;
+    const size_t esz = precision == DQMC_MIXED ? 4 : 8;
+    const int kalloc = delay_k == 1 ? 1 : ((delay_k + 7) / 8) * 8;
+    if (metropolis_lds_bytes(N, kalloc, esz) > 160 * 1024) return DQMC_EINVAL;
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return DQMC_ENODEV;
+
+    dqmc_ctx* c = new dqmc_ctx();
+    c->nl = p->nl; c->N = N; c->L = p->nt; c->pace = p->stabilization_pace;
+    c->S = (c->L % c->pace == 0) ? c->L / c->pace : c->L / c->pace + 1;
+    c->B = n_chains; c->prec = precision; c->kmax = delay_k;
+    c->beta = p->beta; c->dt = p->beta / p->nt; c->nnt = p->nnt; c->u = p->u; c->mu = p->mu;
+    c->alpha = std::acosh(std::exp(0.5 * c->dt * c->u));   // src/hubbard.cpp:35
+    c->eqtime = p->collect_eqtime != 0; c->dynamic = p->collect_dynamic != 0;
+    c->chain_mu.assign(n_chains, p->mu);
+    c->rng_host.assign(n_chains, 1ULL);
+    *out = c;
+
+    const int B = n_chains, L = c->L, S = c->S;
+    const long long nn = c->NN();
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return c->fail(DQMC_EHIP, "stream");
+    HCHECK(c, prepare_kernels());
+    for (auto& e : c->ev) HCHECK(c, hipEventCreate(&e));
+    RCHECK(c->alloc(&c->expK, nn * 8)); RCHECK(c->alloc(&c->invExpK, nn * 8));
+    RCHECK(c->alloc(&c->tab, (size_t)B * 16)); RCHECK(c->alloc(&c->itab, (size_t)B * 16));
+    RCHECK(c->alloc(&c->delta, (size_t)B * 16));
+    RCHECK(c->alloc(&c->powtab, (size_t)2 * N * 8)); RCHECK(c->alloc(&c->rng, (size_t)B * 8));
+    RCHECK(c->alloc(&c->acc_count, (size_t)B * 8));
+    RCHECK(c->alloc(&c->accmask, (size_t)B * L * ((N + 31) / 32) * 4));
+    RCHECK(c->alloc(&c->fields, (size_t)B * L * N));
+    RCHECK(c->alloc(&c->G64, (size_t)B * nn * 8));
+    if (precision == DQMC_MIXED) {
+        RCHECK(c->alloc(&c->G32, (size_t)B * nn * 4)); RCHECK(c->alloc(&c->T32, (size_t)B * nn * 4));
+        RCHECK(c->alloc(&c->expK32, nn * 4)); RCHECK(c->alloc(&c->invExpK32, nn * 4));
+    }
+    RCHECK(c->alloc(&c->stkA, (size_t)B * nn * 8)); RCHECK(c->alloc(&c->stkB, (size_t)B * nn * 8));
+    RCHECK(c->alloc(&c->W1, (size_t)B * nn * 8)); RCHECK(c->alloc(&c->W2, (size_t)B * nn * 8));
+    RCHECK(c->alloc(&c->W3, (size_t)B * nn * 8));
+    RCHECK(c->alloc(&c->qr, (size_t)B * c->sqr() * 8));
+    RCHECK(c->alloc(&c->perm, (size_t)B * N * 4));
+    for (DStack* s : {&c->left, &c->right}) {
+        RCHECK(c->alloc(&s->U, (size_t)B * S * nn * 8));
+        RCHECK(c->alloc(&s->V, (size_t)B * S * nn * 8));
+        RCHECK(c->alloc(&s->D, (size_t)B * S * N * 8));
+    }
+    if (c->eqtime || c->dynamic) RCHECK(c->alloc(&c->vecgftt, (size_t)B * L * nn * 8));
+    if (c->dynamic) {
+        RCHECK(c->alloc(&c->vecgft0, (size_t)B * L * nn * 8)); RCHECK(c->alloc(&c->vecgf0t, (size_t)B * L * nn * 8));
+        RCHECK(c->alloc(&c->gt0A, (size_t)B * nn * 8)); RCHECK(c->alloc(&c->gt0B, (size_t)B * nn * 8));
+        RCHECK(c->alloc(&c->g0tA, (size_t)B * nn * 8)); RCHECK(c->alloc(&c->g0tB, (size_t)B * nn * 8));
+    }
+    RCHECK(c->alloc(&c->wrap_eq, (size_t)B * 8)); RCHECK(c->alloc(&c->wrap_dyn, (size_t)B * 8));
+    RCHECK(c->alloc(&c->maxdG, (size_t)B * 8));
+
+    // constant tables
+    std::vector<double> ek, iek;
+    lattice_expk(c->nl, c->dt * c->nnt, ek);    // exp(-dt K) at mu = 0
+    lattice_expk(c->nl, -c->dt * c->nnt, iek);  // exp(+dt K) at mu = 0
+    HCHECK(c, hipMemcpy(c->expK, ek.data(), nn * 8, hipMemcpyHostToDevice));
+    HCHECK(c, hipMemcpy(c->invExpK, iek.data(), nn * 8, hipMemcpyHostToDevice));
+    if (precision == DQMC_MIXED) {
+        std::vector<float> f(nn);
+        for (long long i = 0; i < nn; ++i) f[i] = (float)ek[i];
+        HCHECK(c, hipMemcpy(c->expK32, f.data(), nn * 4, hipMemcpyHostToDevice));
+        for (long long i = 0; i < nn; ++i) f[i] = (float)iek[i];
+        HCHECK(c, hipMemcpy(c->invExpK32, f.data(), nn * 4, hipMemcpyHostToDevice));
+    }
+    std::vector<uint64_t> pw(2 * N);
+    uint64_t x = 1;
+    for (int k = 0; k < 2 * N; ++k) { x = (x * 16807ULL) % kMinstdM; pw[k] = x; }
+    HCHECK(c, hipMemcpy(c->powtab, pw.data(), pw.size() * 8, hipMemcpyHostToDevice));
+    std::vector<unsigned long long> zero(B, 0ULL);
+    HCHECK(c, hipMemcpy(c->acc_count, zero.data(), (size_t)B * 8, hipMemcpyHostToDevice));
+    std::vector<int8_t> ones((size_t)B * L * N, 1);
+    HCHECK(c, hipMemcpy(c->fields, ones.data(), ones.size(), hipMemcpyHostToDevice));
+    std::vector<uint64_t> r1(B, 1ULL);
+    HCHECK(c, hipMemcpy(c->rng, r1.data(), (size_t)B * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+void dqmc_destroy(dqmc_ctx* c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+    for (void* p : c->allocs) (void)hipFree(p);
+    for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* dqmc_last_error(dqmc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+static int check_chain(dqmc_ctx* c, int chain)
+{
+    if (!c) return DQMC_EINVAL;
+    if (chain < 0 || chain >= c->B) return c->fail(DQMC_EINVAL, "chain %d out of range [0,%d)", chain, c->B);
+    return 0;
+}
+
+int dqmc_set_chain_mu(dqmc_ctx* c, int chain, double mu)
+{
+    RCHECK(check_chain(c, chain));
+    if (c->initialized) return c->fail(DQMC_ESTATE, "set_chain_mu after init");
+    c->chain_mu[chain] = mu;
+    return 0;
+}
+
+int dqmc_set_seed(dqmc_ctx* c, int chain, uint64_t seed)
+{
+    RCHECK(check_chain(c, chain));
+    uint64_t v = seed % kMinstdM;
+    c->rng_host[chain] = v ? v : 1ULL;   // libstdc++ linear_congruential_engine::seed
+    HCHECK(c, hipMemcpy(c->rng + chain, &c->rng_host[chain], 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int dqmc_get_rng_state(dqmc_ctx* c, int chain, uint64_t* state)
+{
+    RCHECK(check_chain(c, chain));
+    if (!state) return DQMC_EINVAL;
+    HCHECK(c, hipMemcpy(state, c->rng + chain, 8, hipMemcpyDeviceToHost));
+    c->rng_host[chain] = *state;
+    return 0;
+}
+
+int dqmc_fields_random(dqmc_ctx* c, int chain)
+{
+    RCHECK(check_chain(c, chain));
+    uint64_t x = 0;
+    HCHECK(c, hipMemcpy(&x, c->rng + chain, 8, hipMemcpyDeviceToHost));
+    std::vector<int8_t> s((size_t)c->L * c->N);
+    for (auto& v : s) v = host_canonical(x) < 0.5 ? 1 : -1;
+    HCHECK(c, hipMemcpy(c->fields + (size_t)chain * c->L * c->N, s.data(), s.size(), hipMemcpyHostToDevice));
+    HCHECK(c, hipMemcpy(c->rng + chain, &x, 8, hipMemcpyHostToDevice));
+    c->rng_host[chain] = x;
+    return 0;
+}
+
+int dqmc_set_fields(dqmc_ctx* c, int chain, const int8_t* s)
+{
+    RCHECK(check_chain(c, chain));
+    if (!s) return DQMC_EINVAL;
+    for (long long k = 0; k < (long long)c->L * c->N; ++k)
+        if (s[k] != 1 && s[k] != -1) return c->fail(DQMC_EINVAL, "field value %d at %lld is not +1/-1", s[k], k);
+    HCHECK(c, hipMemcpy(c->fields + (size_t)chain * c->L * c->N, s, (size_t)c->L * c->N, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int dqmc_get_fields(dqmc_ctx* c, int chain, int8_t* s)
+{
+    RCHECK(check_chain(c, chain));
+    if (!s) return DQMC_EINVAL;
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(s, c->fields + (size_t)chain * c->L * c->N, (size_t)c->L * c->N, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_init(dqmc_ctx* c)
+{
+    if (!c) return DQMC_EINVAL;
+    const int B = c->B, N = c->N, L = c->L;
+    const long long nn = c->NN();
+    // per-chain scale tables: B = diag(e^{dt mu + alpha s}) exp(dt nnt A), B^{-1} = exp(-dt nnt A) diag(e^{-dt mu - alpha s})
+    std::vector<double> tab(2 * B), itab(2 * B), del(2 * B);
+    for (int b = 0; b < B; ++b) {
+        const double m = c->dt * c->chain_mu[b];
+        tab[2 * b] = std::exp(m + c->alpha); tab[2 * b + 1] = std::exp(m - c->alpha);
+        itab[2 * b] = std::exp(-m - c->alpha); itab[2 * b + 1] = std::exp(-m + c->alpha);
+        del[2 * b] = std::exp(-2 * c->alpha * (+1)) - 1;   // src/hubbard.cpp:127
+        del[2 * b + 1] = std::exp(-2 * c->alpha * (-1)) - 1;
+    }
+    HCHECK(c, hipMemcpy(c->tab, tab.data(), 16 * (size_t)B, hipMemcpyHostToDevice));
+    HCHECK(c, hipMemcpy(c->itab, itab.data(), 16 * (size_t)B, hipMemcpyHostToDevice));
+    HCHECK(c, hipMemcpy(c->delta, del.data(), 16 * (size_t)B, hipMemcpyHostToDevice));
+    HCHECK(c, launch_fill_d(c->wrap_eq, B, 0.0, c->stream));
+    HCHECK(c, launch_fill_d(c->wrap_dyn, B, 0.0, c->stream));
+    HCHECK(c, launch_fill_d(c->maxdG, B, 0.0, c->stream));
+    if (!c->initialized) RCHECK(autotune(c));
+    // Core::initialize_svdstacks (src/dqmc_core.cpp:67-87)
+    c->left.len = 0; c->right.len = 0; c->current_t = 0;
+    RCHECK(set_identity_stk(c));
+    for (int t = L; t >= 1; --t) {
+        RCHECK(chain_mul(c, t % L, true));
+        if ((t - 1) % c->pace == 0) {
+            RCHECK(stack_push(c, c->right));
+            RCHECK(set_identity_stk(c));
+        }
+    }
+    // Core::initialize_green_functions (:89-97)
+    RCHECK(recompute_eq(c, c->W3));
+    HCHECK(c, launch_copy_d(c->W3, nn, c->G64, nn, nn, B, c->stream));
+    if (c->prec == DQMC_MIXED) HCHECK(c, launch_copy_d2f(c->W3, nn, c->G32, nn, nn, B, c->stream));
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    (void)N;
+    c->initialized = true;
+    return 0;
+}
+
+int dqmc_apply_bop(dqmc_ctx* c, int chain, int op, int t, double* inout)
+{
+    RCHECK(check_chain(c, chain));
+    if (!inout || op < 0 || op > 4 || t < 0 || t >= c->L) return c->fail(DQMC_EINVAL, "bad apply_bop arguments");
+    if (!c->initialized) return c->fail(DQMC_ESTATE, "apply_bop before init");
+    const long long nn = c->NN();
+    const int N = c->N;
+    // every chain computes on W2 (only `chain`'s result is read back)
+    HCHECK(c, hipMemcpy(c->W2 + chain * nn, inout, nn * 8, hipMemcpyHostToDevice));
+    switch (op) {
+        case 0:  // B G = D (expK G)
+            RCHECK(gemm<double>(c, c->expK, 0, N, 0, c->W2, nn, N, 0, c->W3, nn, N, 0, dfield(c, c->tab, t)));
+            break;
+        case 1:  // G B = (G D) expK
+            RCHECK(gemm<double>(c, c->W2, nn, N, 0, c->expK, 0, N, 0, c->W3, nn, N, 0, dnone(), dfield(c, c->tab, t)));
+            break;
+        case 2:  // B^{-1} G = invExpK (D^{-1} G)
+            RCHECK(gemm<double>(c, c->invExpK, 0, N, 0, c->W2, nn, N, 0, c->W3, nn, N, 0, dnone(), dfield(c, c->itab, t)));
+            break;
+        case 3:  // G B^{-1} = (G invExpK) D^{-1}
+            RCHECK(gemm<double>(c, c->W2, nn, N, 0, c->invExpK, 0, N, 0, c->W3, nn, N, 0, dnone(), dnone(),
+                                dfield(c, c->itab, t)));
+            break;
+        default:  // B^T G = expK (D G)
+            RCHECK(gemm<double>(c, c->expK, 0, N, 0, c->W2, nn, N, 0, c->W3, nn, N, 0, dnone(), dfield(c, c->tab, t)));
+            break;
+    }
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(inout, c->W3 + chain * nn, nn * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_set_thermalization(dqmc_ctx* c, int th)
+{
+    if (!c) return DQMC_EINVAL;
+    c->thermalization = th != 0;
+    return 0;
+}
+
+int dqmc_sweep_up(dqmc_ctx* c)
+{
+    if (!c) return DQMC_EINVAL;
+    if (!c->initialized || c->current_t != 0 || c->left.len != 0 || c->right.len != c->S)
+        return c->fail(DQMC_ESTATE, "sweep_up needs current_t == 0, empty left and full right stacks");
+    RCHECK(run_sweep(c, do_sweep_up, 0));
+    c->current_t = c->L;
+    c->updating_sweeps += 1;
+    return 0;
+}
+
+int dqmc_sweep_down(dqmc_ctx* c)
+{
+    if (!c) return DQMC_EINVAL;
+    if (!c->initialized || c->current_t != c->L || c->right.len != 0 || c->left.len != c->S)
+        return c->fail(DQMC_ESTATE, "sweep_down needs current_t == nt, empty right and full left stacks");
+    RCHECK(run_sweep(c, do_sweep_down, 1));
+    c->current_t = 0;
+    c->updating_sweeps += 1;
+    return 0;
+}
+
+int dqmc_sweep_dynamic(dqmc_ctx* c)
+{
+    if (!c) return DQMC_EINVAL;
+    if (!c->dynamic) return 0;   // Core::sweep_for_dynamic_green_functions is a no-op then
+    if (!c->initialized || c->current_t != 0 || c->left.len != 0 || c->right.len != c->S)
+        return c->fail(DQMC_ESTATE, "sweep_dynamic needs current_t == 0, empty left and full right stacks");
+    RCHECK(run_sweep(c, do_sweep_dynamic, 2));
+    c->current_t = c->L;
+    return 0;
+}
+
+int dqmc_get_gf(dqmc_ctx* c, int chain, int kind, int t, double* out)
+{
+    RCHECK(check_chain(c, chain));
+    if (!out) return DQMC_EINVAL;
+    const long long nn = c->NN();
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    if (kind == DQMC_GF_CURRENT) {
+        if (c->prec == DQMC_MIXED) {
+            std::vector<float> f(nn);
+            HCHECK(c, hipMemcpy(f.data(), c->G32 + chain * nn, nn * 4, hipMemcpyDeviceToHost));
+            for (long long i = 0; i < nn; ++i) out[i] = f[i];
+        } else {
+            HCHECK(c, hipMemcpy(out, c->G64 + chain * nn, nn * 8, hipMemcpyDeviceToHost));
+        }
+        return 0;
+    }
+    if (t < 0 || t >= c->L) return c->fail(DQMC_EINVAL, "t out of range");
+    const double* src = kind == DQMC_GF_TT ? c->vecgftt : kind == DQMC_GF_T0 ? c->vecgft0 : kind == DQMC_GF_0T ? c->vecgf0t : nullptr;
+    if (!src) return c->fail(DQMC_EINVAL, "Green's function kind %d not collected", kind);
+    HCHECK(c, hipMemcpy(out, src + ((long long)chain * c->L + t) * nn, nn * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_set_gf(dqmc_ctx* c, int chain, const double* g)
+{
+    RCHECK(check_chain(c, chain));
+    if (!g) return DQMC_EINVAL;
+    const long long nn = c->NN();
+    HCHECK(c, hipMemcpy(c->G64 + chain * nn, g, nn * 8, hipMemcpyHostToDevice));
+    if (c->prec == DQMC_MIXED) {
+        std::vector<float> f(nn);
+        for (long long i = 0; i < nn; ++i) f[i] = (float)g[i];
+        HCHECK(c, hipMemcpy(c->G32 + chain * nn, f.data(), nn * 4, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int dqmc_get_stats(dqmc_ctx* c, int chain, double* wrap_eq, double* wrap_dyn, double* max_abs_dG, long long* accepts,
+                   long long* visits)
+{
+    RCHECK(check_chain(c, chain));
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    double v;
+    if (wrap_eq) { HCHECK(c, hipMemcpy(&v, c->wrap_eq + chain, 8, hipMemcpyDeviceToHost)); *wrap_eq = v; }
+    if (wrap_dyn) { HCHECK(c, hipMemcpy(&v, c->wrap_dyn + chain, 8, hipMemcpyDeviceToHost)); *wrap_dyn = v; }
+    if (max_abs_dG) { HCHECK(c, hipMemcpy(&v, c->maxdG + chain, 8, hipMemcpyDeviceToHost)); *max_abs_dG = v; }
+    unsigned long long a = 0;
+    HCHECK(c, hipMemcpy(&a, c->acc_count + chain, 8, hipMemcpyDeviceToHost));
+    if (accepts) *accepts = (long long)a;
+    if (visits) *visits = c->updating_sweeps * (long long)c->L * c->N;
+    return 0;
+}
+
+int dqmc_get_accept_mask(dqmc_ctx* c, int chain, uint32_t* out)
+{
+    RCHECK(check_chain(c, chain));
+    if (!out) return DQMC_EINVAL;
+    const size_t n = (size_t)c->L * ((c->N + 31) / 32);
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(out, c->accmask + chain * n, n * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_synchronize(dqmc_ctx* c)
+{
+    if (!c) return DQMC_EINVAL;
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dqmc_event_record(dqmc_ctx* c, int slot)
+{
+    if (!c || slot < 0 || slot >= 8) return DQMC_EINVAL;
+    HCHECK(c, hipEventRecord(c->ev[slot], c->stream));
+    return 0;
+}
+
+int dqmc_event_elapsed(dqmc_ctx* c, int a, int b, float* ms)
+{
+    if (!c || a < 0 || a >= 8 || b < 0 || b >= 8 || !ms) return DQMC_EINVAL;
+    HCHECK(c, hipEventSynchronize(c->ev[b]));
+    HCHECK(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return 0;
+}
+
+int dqmc_use_graphs(dqmc_ctx* c, int on)
+{
+    if (!c) return DQMC_EINVAL;
+    c->use_graphs = on != 0;
+    return 0;
+}
+
+int dqmc_dims(dqmc_ctx* c, int* ns, int* nt, int* nch)
+{
+    if (!c) return DQMC_EINVAL;
+    if (ns) *ns = c->N;
+    if (nt) *nt = c->L;
+    if (nch) *nch = c->B;
+    return 0;
+}
+
+}  // extern "C"

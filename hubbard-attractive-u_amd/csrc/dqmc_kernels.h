@@ -1,0 +1,128 @@
+// dqmc_kernels.h -- launchers of the gfx950 kernels of the DQMC sweep engine.
+//
+// Layout conventions (device):
+//   * N x N matrices are column-major with leading dimension ld, batched over chains with a
+//     fixed element stride between consecutive chains.
+//   * Matrices handed to the Householder QR are row-major: element (r, c) at buf[r*W + c]
+//     with W = N + P (the P extra columns carry right-hand sides that receive Q^T * rhs).
+//   * Fields are int8 [chain][t][i].
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dqmc {
+
+// Diagonal scaling operand of a GEMM: value for index i of chain b.
+enum DiagMode : int {
+    DIAG_NONE = 0,
+    DIAG_VEC = 1,       // v[b*vstride + i]
+    DIAG_INV = 2,       // 1 / v[...]
+    DIAG_INVMAX1 = 3,   // 1 / max(1, v[...])
+    DIAG_MIN1 = 4,      // min(1, v[...])
+    DIAG_FIELD = 5      // f[b*fstride + i] > 0 ? v[b*vstride + 0] : v[b*vstride + 1]
+};
+
+struct DiagArg {
+    int mode = DIAG_NONE;
+    const double* v = nullptr;
+    long long vstride = 0;
+    const int8_t* f = nullptr;
+    long long fstride = 0;
+};
+
+template <typename T>
+struct GemmArgs {
+    int M, Nc, K;
+    const T* A; long long sA; int lda; int tA;   // op(A) is M x K
+    const T* B; long long sB; int ldb; int tB;   // op(B) is K x Nc
+    T* C; long long sC; int ldc; int tC;         // tC: store C^T (element (i,j) at j + i*ldc... see kernel)
+    double alpha, beta;                          // C = alpha * diag(r) op(A) diag(s) op(B) diag(c) + beta * C
+    DiagArg rowd, innd, cold;
+    int batch;
+};
+
+// Tile shapes the autotuner chooses from.
+enum GemmTile : int { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_COUNT = 3 };
+
+template <typename T>
+hipError_t launch_gemm(const GemmArgs<T>& a, int tile, hipStream_t s);
+
+// Metropolis sweep of one slice with delayed rank-k updates.
+template <typename T>
+struct MetroArgs {
+    T* G; long long sG;               // working G, column-major N x N per chain
+    int8_t* fields;                   // [chain][L][N]
+    int N, L, t;
+    uint64_t* rng;                    // [chain] minstd_rand0 state
+    const uint64_t* powtab;           // 16807^(k+1) mod (2^31-1), k = 0..2N-1
+    const double* delta;              // [chain][2]: e^{-2 alpha s} - 1 for s = +1, -1
+    uint32_t* accmask;                // [chain][L][ceil(N/32)]
+    unsigned long long* acc_count;    // [chain]
+    int kmax;
+};
+template <typename T>
+hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, hipStream_t s);
+// LDS bytes of one Metropolis workgroup (krows pending-update rows of X and Y).
+size_t metropolis_lds_bytes(int N, int krows, size_t esz);
+// One-time kernel attribute setup (large dynamic LDS); call before any graph capture.
+hipError_t prepare_kernels();
+
+// Householder QR (no pivoting) of the leading N x N block of a row-major N x W buffer,
+// applying Q^T to the trailing W-N columns.  On exit the leading block holds R (zeros
+// below the diagonal).
+hipError_t launch_qr(double* buf, long long stride, int N, int W, int batch, hipStream_t s);
+
+// In-place X = R^{-1} Z, R the upper triangle of the leading N x N block, Z columns
+// [c0, c0+P) of the same row-major N x W buffer.
+hipError_t launch_trsm_upper(double* buf, long long stride, int N, int W, int c0, int P, int batch,
+                             hipStream_t s);
+
+// Pre-pivoted UDV input: M column-major (ld N); writes buf[r][j] = M(r, perm[j]) with perm
+// sorting the columns of M by descending 2-norm, and buf[r][N+j] = delta_rj.
+hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long long sbuf, int* perm,
+                              int N, int batch, hipStream_t s);
+// After QR: D_i = |R_ii|, Tt (column-major N x N) with Tt(perm[j], i) = R_ij / D_i (j >= i),
+// U (column-major) = Q read from the trailing block.
+hipError_t launch_udv_finish(const double* buf, long long sbuf, const int* perm, double* U, long long sU,
+                             double* D, long long sD, double* Tt, long long sTt, int N, int batch,
+                             hipStream_t s);
+
+// Builds the row-major system for (1 + u diag(d) v^T)^{-1}:
+//   buf[i][j] = sl_i u(j,i) + sr_i v(j,i),  buf[i][N+j] = sl_i u(j,i),
+// with sl = 1/max(1,d), sr = min(1,d).
+hipError_t launch_build_00bb(const double* u, long long su, const double* v, long long sv, const double* d,
+                             long long sd, double* buf, long long sbuf, int N, int W, int batch,
+                             hipStream_t s);
+// buf[r][c0 + j] = alpha * scale(r) * X(j, r) for a column-major X (i.e. the scaled transpose).
+hipError_t launch_build_rhs(const double* X, long long sX, DiagArg d, double alpha, double* buf, long long sbuf,
+                            int N, int W, int c0, int batch, hipStream_t s);
+// Copies block columns [c0, c0+N) of the row-major buffer into a column-major matrix;
+// transpose = 0 gives out(j, r) = buf[r][c0+j] (the block is stored transposed), 1 gives
+// out(r, j) = buf[r][c0+j].
+hipError_t launch_extract(const double* buf, long long sbuf, int W, int c0, double* out, long long so, int N,
+                          int transpose, int batch, hipStream_t s);
+
+// Signed-max wrap error and max-abs difference of a new FP64 G against the working copy;
+// then commits new -> G64 (and G32 if non-null).  err_signed / err_abs are per-chain
+// running maxima (updated in place).  sign_mode: 0 = plain, 1 = gt0 = I - g, 2 = g0t = -g, etc. are
+// handled by callers.
+hipError_t launch_compare_commit(const double* Gnew, double* G64, float* G32, long long sG, int N,
+                                 double* err_signed, double* err_abs, int batch, hipStream_t s);
+// Same but the working matrix is FP64 only (dynamic G(t,0), G(0,t)).
+hipError_t launch_compare_only(const double* Gnew, const double* Gcur, long long sG, int N, double* err_signed,
+                               int batch, hipStream_t s);
+
+// Elementwise helpers (n elements per chain).
+hipError_t launch_set_identity(double* A, long long s, int N, double diagval, int batch, hipStream_t st);
+hipError_t launch_copy_d(const double* src, long long ss, double* dst, long long sd, long long n, int batch,
+                         hipStream_t st);
+hipError_t launch_copy_f2d(const float* src, long long ss, double* dst, long long sd, long long n, int batch,
+                           hipStream_t st);
+hipError_t launch_copy_d2f(const double* src, long long ss, float* dst, long long sd, long long n, int batch,
+                           hipStream_t st);
+// dst = a*src + b*I (N x N)
+hipError_t launch_axpi(const double* src, long long ss, double* dst, long long sd, double a, double b, int N,
+                       int batch, hipStream_t st);
+hipError_t launch_fill_d(double* dst, long long n, double v, hipStream_t st);
+
+}  // namespace dqmc

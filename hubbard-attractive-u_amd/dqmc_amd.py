@@ -1,0 +1,204 @@
+"""Python binding of the MI355X DQMC sweep engine (libdqmc_amd.so, C ABI in include/dqmc_amd.h).
+
+`Engine` mirrors the sweep-path interface of the reference's DQMC::Core and
+Model::HubbardAttractiveU (include/dqmc_core.h:76-98, include/hubbard.h:58-73) for a batch
+of independent chains.  The shared library must have been built (see Makefile / build());
+loading fails loudly when it is missing - there is no CPU fallback.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdqmc_amd.so")
+
+FP64, MIXED = 0, 1
+GF_CURRENT, GF_TT, GF_T0, GF_0T = 0, 1, 2, 3
+_ERRS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "ENUMERIC", -5: "ENOMEM", -6: "ENODEV"}
+
+
+class DqmcError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    """dqmc_params_c: the config.toml keys of the sweep path (include/dqmc_params.hpp:20-45)."""
+    _fields_ = [("nl", C.c_int), ("nt", C.c_int), ("beta", C.c_double), ("nnt", C.c_double),
+                ("u", C.c_double), ("mu", C.c_double), ("stabilization_pace", C.c_int),
+                ("is_fft", C.c_int), ("collect_eqtime", C.c_int), ("collect_dynamic", C.c_int)]
+
+
+_lib = None
+
+
+def build(force=False):
+    args = ["make", "-C", _HERE]
+    if force:
+        args.append("-B")
+    subprocess.check_call(args)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise DqmcError("libdqmc_amd.so is not built (%s); run `make -C %s`" % (LIB_PATH, _HERE))
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        sig = {
+            "dqmc_create": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+            "dqmc_destroy": (None, [vp]),
+            "dqmc_last_error": (C.c_char_p, [vp]),
+            "dqmc_set_chain_mu": (C.c_int, [vp, C.c_int, C.c_double]),
+            "dqmc_set_seed": (C.c_int, [vp, C.c_int, C.c_uint64]),
+            "dqmc_get_rng_state": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint64)]),
+            "dqmc_fields_random": (C.c_int, [vp, C.c_int]),
+            "dqmc_set_fields": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int8)]),
+            "dqmc_get_fields": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int8)]),
+            "dqmc_init": (C.c_int, [vp]),
+            "dqmc_apply_bop": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
+            "dqmc_set_thermalization": (C.c_int, [vp, C.c_int]),
+            "dqmc_sweep_up": (C.c_int, [vp]),
+            "dqmc_sweep_down": (C.c_int, [vp]),
+            "dqmc_sweep_dynamic": (C.c_int, [vp]),
+            "dqmc_get_gf": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
+            "dqmc_set_gf": (C.c_int, [vp, C.c_int, dp]),
+            "dqmc_get_stats": (C.c_int, [vp, C.c_int, dp, dp, dp, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+            "dqmc_get_accept_mask": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint32)]),
+            "dqmc_synchronize": (C.c_int, [vp]),
+            "dqmc_event_record": (C.c_int, [vp, C.c_int]),
+            "dqmc_event_elapsed": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
+            "dqmc_use_graphs": (C.c_int, [vp, C.c_int]),
+            "dqmc_dims": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class Engine:
+    """A batch of `n_chains` DQMC chains on one GPU."""
+
+    def __init__(self, nl, nt, beta, u, mu=0.0, nnt=1.0, pace=10, n_chains=1, precision=FP64, delay_k=1,
+                 eqtime=False, dynamic=False):
+        self.nl, self.n, self.nt, self.n_chains = nl, nl * nl, nt, n_chains
+        p = Params(nl, nt, beta, nnt, u, mu, pace, 0, int(eqtime), int(dynamic))
+        h = C.c_void_p()
+        rc = lib().dqmc_create(C.byref(p), n_chains, precision, delay_k, C.byref(h))
+        self._h = h
+        if rc:
+            msg = lib().dqmc_last_error(h) if h.value else b""
+            if h.value:
+                lib().dqmc_destroy(h)
+            self._h = C.c_void_p()
+            raise DqmcError("dqmc_create failed: %s %s" % (_ERRS.get(rc, rc), (msg or b"").decode()))
+
+    def _chk(self, rc, what):
+        if rc:
+            raise DqmcError("%s failed: %s %s" % (what, _ERRS.get(rc, rc), lib().dqmc_last_error(self._h).decode()))
+
+    def close(self):
+        if self._h and self._h.value:
+            lib().dqmc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- configuration -----------------------------------------------------------------------
+    def set_chain_mu(self, chain, mu):
+        self._chk(lib().dqmc_set_chain_mu(self._h, chain, mu), "set_chain_mu")
+
+    def set_seed(self, chain, seed):
+        self._chk(lib().dqmc_set_seed(self._h, chain, int(seed)), "set_seed")
+
+    def rng_state(self, chain):
+        x = C.c_uint64()
+        self._chk(lib().dqmc_get_rng_state(self._h, chain, C.byref(x)), "get_rng_state")
+        return int(x.value)
+
+    def fields_random(self, chain):
+        self._chk(lib().dqmc_fields_random(self._h, chain), "fields_random")
+
+    def set_fields(self, chain, s):
+        s = np.ascontiguousarray(np.asarray(s).reshape(self.nt, self.n), dtype=np.int8)
+        self._chk(lib().dqmc_set_fields(self._h, chain, s.ctypes.data_as(C.POINTER(C.c_int8))), "set_fields")
+
+    def fields(self, chain):
+        s = np.zeros((self.nt, self.n), dtype=np.int8)
+        self._chk(lib().dqmc_get_fields(self._h, chain, s.ctypes.data_as(C.POINTER(C.c_int8))), "get_fields")
+        return s
+
+    def use_graphs(self, on=True):
+        self._chk(lib().dqmc_use_graphs(self._h, int(on)), "use_graphs")
+
+    # --- sweep path ----------------------------------------------------------------------------
+    def init(self):
+        self._chk(lib().dqmc_init(self._h), "init")
+
+    def set_thermalization(self, on):
+        self._chk(lib().dqmc_set_thermalization(self._h, int(on)), "set_thermalization")
+
+    def sweep_up(self):
+        self._chk(lib().dqmc_sweep_up(self._h), "sweep_up")
+
+    def sweep_down(self):
+        self._chk(lib().dqmc_sweep_down(self._h), "sweep_down")
+
+    def sweep_dynamic(self):
+        self._chk(lib().dqmc_sweep_dynamic(self._h), "sweep_dynamic")
+
+    def apply_bop(self, chain, op, t, g):
+        g = np.array(g, dtype=np.float64, order="F", copy=True)
+        self._chk(lib().dqmc_apply_bop(self._h, chain, op, t, g.ctypes.data_as(C.POINTER(C.c_double))), "apply_bop")
+        return g
+
+    # --- results -------------------------------------------------------------------------------
+    def gf(self, chain, kind=GF_CURRENT, t=0):
+        out = np.zeros((self.n, self.n), order="F")
+        self._chk(lib().dqmc_get_gf(self._h, chain, kind, t, out.ctypes.data_as(C.POINTER(C.c_double))), "get_gf")
+        return out
+
+    def set_gf(self, chain, g):
+        g = np.asfortranarray(g, dtype=np.float64)
+        self._chk(lib().dqmc_set_gf(self._h, chain, g.ctypes.data_as(C.POINTER(C.c_double))), "set_gf")
+
+    def stats(self, chain):
+        we, wd, dg = C.c_double(), C.c_double(), C.c_double()
+        a, v = C.c_longlong(), C.c_longlong()
+        self._chk(lib().dqmc_get_stats(self._h, chain, C.byref(we), C.byref(wd), C.byref(dg), C.byref(a),
+                                       C.byref(v)), "get_stats")
+        return {"wrap_eq": we.value, "wrap_dyn": wd.value, "max_abs_dG": dg.value, "accepts": a.value,
+                "visits": v.value}
+
+    def accept_mask(self, chain):
+        out = np.zeros((self.nt, (self.n + 31) // 32), dtype=np.uint32)
+        self._chk(lib().dqmc_get_accept_mask(self._h, chain, out.ctypes.data_as(C.POINTER(C.c_uint32))),
+                  "get_accept_mask")
+        return out
+
+    def synchronize(self):
+        self._chk(lib().dqmc_synchronize(self._h), "synchronize")
+
+    def event_record(self, slot):
+        self._chk(lib().dqmc_event_record(self._h, slot), "event_record")
+
+    def event_elapsed(self, a, b):
+        ms = C.c_float()
+        self._chk(lib().dqmc_event_elapsed(self._h, a, b, C.byref(ms)), "event_elapsed")
+        return float(ms.value)

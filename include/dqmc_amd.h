@@ -1,0 +1,140 @@
+/*
+ * dqmc_amd.h -- C ABI of the MI355X (gfx950) DQMC sweep engine.
+ *
+ * The engine replaces the reference's sweep path: DQMC::Core (include/dqmc_core.h:35-112)
+ * driving Model::HubbardAttractiveU (include/hubbard.h:31-111) with Utils::SvdStack and
+ * Utils::StableNumerics underneath.  One context holds `n_chains` independent Markov
+ * chains (the reference runs one chain per MPI rank, src/main.cpp:111) resident on one GPU.
+ *
+ * Conventions
+ *   - Every call returns 0 on success or a negative DQMC_E* code; no exception crosses
+ *     the ABI.  dqmc_last_error() returns the message of the last failure of a context.
+ *   - The caller owns all host buffers.  The context owns device memory and one HIP stream.
+ *   - Matrices crossing the ABI are N x N FP64, column-major (Eigen's default, the layout
+ *     of the reference's Eigen::MatrixXd, include/dqmc_core.h:27).
+ *   - Fields cross the ABI as int8 in [t][i] order (nt x ns), the row order of the
+ *     reference's `ising.fields` file (include/dqmc_io.hpp:234-255).
+ *   - Calls are stream-ordered and synchronous at return unless stated otherwise.
+ *   - One host thread per context.
+ */
+#ifndef DQMC_AMD_H
+#define DQMC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dqmc_ctx dqmc_ctx;
+
+/* Mirrors DQMC::Params (include/dqmc_params.hpp:20-45) for the fields the sweep path reads
+ * (keys parsed at include/dqmc_parser.hpp:60-96). */
+typedef struct dqmc_params_c {
+    int nl;                  /* [Lattice.SquareLattice] nl, ns = ng = nl*nl          */
+    int nt;                  /* [MonteCarlo] nt                                      */
+    double beta;             /* [MonteCarlo] beta, dt = beta / nt                    */
+    double nnt;              /* [Model.HubbardAttractiveU] nnt                       */
+    double u;                /* [Model.HubbardAttractiveU] u (absolute value of U)   */
+    double mu;               /* [Model.HubbardAttractiveU] mu (default of each chain)*/
+    int stabilization_pace;  /* [MonteCarlo] stabilization_pace                      */
+    int is_fft;              /* [MonteCarlo] is_fft: accepted, the dense path is used */
+    int collect_eqtime;      /* keep G(t,t) of every slice (Measurement::Handle::isEqualTime) */
+    int collect_dynamic;     /* keep G(t,0), G(0,t) (Measurement::Handle::isDynamic)  */
+} dqmc_params_c;
+
+enum { DQMC_FP64 = 0, DQMC_MIXED = 1 };
+enum { DQMC_GF_CURRENT = 0, DQMC_GF_TT = 1, DQMC_GF_T0 = 2, DQMC_GF_0T = 3 };
+enum {
+    DQMC_OK = 0,
+    DQMC_EINVAL = -1,     /* bad argument                                   */
+    DQMC_EHIP = -2,       /* HIP runtime error                              */
+    DQMC_ESTATE = -3,     /* call not allowed in the current sweep state    */
+    DQMC_ENUMERIC = -4,   /* NaN/Inf or a zero pivot in a stabilization     */
+    DQMC_ENOMEM = -5,
+    DQMC_ENODEV = -6      /* no gfx950 device                               */
+};
+
+/* Context lifetime.  precision: DQMC_FP64 (everything FP64) or DQMC_MIXED (wraps and the
+ * delayed-update flushes in FP32, stabilization / recompose / accept ratios FP64).
+ * delay_k: 1 = one Sherman-Morrison rank-1 update per accepted flip
+ * (src/hubbard.cpp:132-139); k > 1 regroups the same updates into rank-k flushes.
+ * Replaces: construction of DQMC::Core + Model::HubbardAttractiveU + SquareLattice and their
+ * initialize() calls (src/main.cpp:119-128; src/dqmc_core.cpp:20-30; src/hubbard.cpp:25-48). */
+int dqmc_create(const dqmc_params_c* params, int n_chains, int precision, int delay_k, dqmc_ctx** out);
+void dqmc_destroy(dqmc_ctx* ctx);
+const char* dqmc_last_error(dqmc_ctx* ctx);
+
+/* Per-chain chemical potential (the batched workload spreads mu over chains).
+ * Must be called before dqmc_init. */
+int dqmc_set_chain_mu(dqmc_ctx* ctx, int chain, double mu);
+
+/* Replaces Utils::Random::set_seed (src/random.cpp:15): each chain owns one
+ * std::default_random_engine (minstd_rand0) stream, seeded with libstdc++ semantics. */
+int dqmc_set_seed(dqmc_ctx* ctx, int chain, uint64_t seed);
+int dqmc_get_rng_state(dqmc_ctx* ctx, int chain, uint64_t* state);
+
+/* Replaces Model::HubbardAttractiveU::set_ising_fields_to_random (src/hubbard.cpp:102-110):
+ * bernoulli(0.5) -> +1/-1 in t-major order, drawn from the chain's stream. */
+int dqmc_fields_random(dqmc_ctx* ctx, int chain);
+/* Replaces DQMC::IO::load/save_ising_fields_configs' access to m_ising_fields
+ * (include/dqmc_io.hpp:234-290).  s: nt*ns int8 in [t][i] order, values +1/-1. */
+int dqmc_set_fields(dqmc_ctx* ctx, int chain, const int8_t* s);
+int dqmc_get_fields(dqmc_ctx* ctx, int chain, int8_t* s);
+
+/* Replaces Core::initialize_svdstacks + Core::initialize_green_functions
+ * (src/dqmc_core.cpp:67-97) for all chains. */
+int dqmc_init(dqmc_ctx* ctx);
+/* Replaces Model::HubbardAttractiveU::multiply_B_from_left / multiply_B_from_right /
+ * multiply_invB_from_left / multiply_invB_from_right / multiply_adjB_from_left
+ * (include/hubbard.h:58-62, src/hubbard.cpp:293-341) for ops 0..4, applied in place to an
+ * N x N column-major FP64 matrix with chain `chain`'s fields of slice t (FP64 path). */
+int dqmc_apply_bop(dqmc_ctx* ctx, int chain, int op, int t, double* inout);
+
+/* Replaces Core::set_thermalization (include/dqmc_core.h:90). */
+int dqmc_set_thermalization(dqmc_ctx* ctx, int is_thermalization);
+
+/* One directional sweep of every chain.
+ * Replace Core::sweep_from_0_to_beta (src/dqmc_core.cpp:167-227),
+ * Core::sweep_from_beta_to_0 (:233-284) and
+ * Core::sweep_for_dynamic_green_functions (:291-390). */
+int dqmc_sweep_up(dqmc_ctx* ctx);
+int dqmc_sweep_down(dqmc_ctx* ctx);
+int dqmc_sweep_dynamic(dqmc_ctx* ctx);
+
+/* Replaces the accessors Core::current_gftt / gftt(t) / gft0(t) / gf0t(t)
+ * (include/dqmc_core.h:76-79).  out: N*N FP64 column-major.  For DQMC_MIXED the current
+ * G is the FP32 working copy widened to FP64. */
+int dqmc_get_gf(dqmc_ctx* ctx, int chain, int kind, int t, double* out);
+/* Overwrite a chain's current G(t,t) (FP64 and the FP32 working copy); test hook. */
+int dqmc_set_gf(dqmc_ctx* ctx, int chain, const double* g);
+
+/* Replaces Core::equaltimeWrapError / dynamicWrapError (include/dqmc_core.h:80-81); the
+ * error metric is the reference's signed max (include/utils/stable_numerics.hpp:42).
+ * max_abs_dG: max over the run of max_ij |G_working - G_FP64| at each stabilization, with
+ * G_FP64 recomputed from the same fields (the mixed-precision error; equals the absolute
+ * wrap error in FP64 mode). */
+int dqmc_get_stats(dqmc_ctx* ctx, int chain, double* wrap_eq, double* wrap_dyn, double* max_abs_dG,
+                   long long* accepts, long long* visits);
+/* Accept bits of the last updating sweep: nt x ceil(ns/32) uint32, bit i%32 of word
+ * [t][i/32] set when the flip at (t, i) was accepted (src/dqmc_core.cpp:110). */
+int dqmc_get_accept_mask(dqmc_ctx* ctx, int chain, uint32_t* out);
+
+/* Wall-clock timing helpers for benchmarks: record events around the next sweeps on the
+ * context's stream and return the elapsed milliseconds between two marks. */
+int dqmc_synchronize(dqmc_ctx* ctx);
+int dqmc_event_record(dqmc_ctx* ctx, int slot);
+int dqmc_event_elapsed(dqmc_ctx* ctx, int slot_begin, int slot_end, float* ms);
+
+/* Capture each sweep kind into a hipGraph on first use and replay it afterwards
+ * (1 = on, default 0). */
+int dqmc_use_graphs(dqmc_ctx* ctx, int on);
+
+/* Lattice size / chain count accessors. */
+int dqmc_dims(dqmc_ctx* ctx, int* ns, int* nt, int* n_chains);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DQMC_AMD_H */
