@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -60,6 +61,8 @@ struct dqmc_ctx {
     double dt = 0, beta = 0, nnt = 0, u = 0, mu = 0, alpha = 0;
     bool eqtime = false, dynamic = false, thermalization = false, use_graphs = false;
     bool initialized = false;
+    // blocked QR / LDS triangular solve / Kronecker wraps: opt-in (DQMC_EXPERIMENTAL_KERNELS=1)
+    bool experimental = false;
     int current_t = 0;
     long long updating_sweeps = 0;
     std::vector<double> chain_mu;
@@ -69,12 +72,17 @@ struct dqmc_ctx {
 
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
+    // per-launch kernel timer (benchmark instrumentation)
+    int kt_kernel = -1, kt_used = 0;
+    std::vector<hipEvent_t> kt_ev;
     std::map<int, hipGraphExec_t> graphs;
 
     // device memory
     std::vector<void*> allocs;
     double *expK = nullptr, *invExpK = nullptr;
     float *expK32 = nullptr, *invExpK32 = nullptr;
+    double *E1 = nullptr, *F1 = nullptr;       // 16x16 padded exp(+-dt nnt A1) of the ring
+    float *E1f = nullptr, *F1f = nullptr;
     double *tab = nullptr, *itab = nullptr, *delta = nullptr;
     uint64_t *powtab = nullptr, *rng = nullptr;
     unsigned long long* acc_count = nullptr;
@@ -128,6 +136,27 @@ struct dqmc_ctx {
 
 namespace {
 
+enum { KT_METRO = 0, KT_QR = 1, KT_GEMM = 2, KT_TRSM = 3, KT_KRON = 4 };
+int kt_pre(dqmc_ctx* c, int id)
+{
+    if (c->kt_kernel != id || c->kt_used + 2 > (int)c->kt_ev.size()) return 0;
+    HCHECK(c, hipEventRecord(c->kt_ev[c->kt_used], c->stream));
+    return 0;
+}
+int kt_post(dqmc_ctx* c, int id)
+{
+    if (c->kt_kernel != id || c->kt_used + 2 > (int)c->kt_ev.size()) return 0;
+    HCHECK(c, hipEventRecord(c->kt_ev[c->kt_used + 1], c->stream));
+    c->kt_used += 2;
+    return 0;
+}
+#define TIMED(c, id, call)          \
+    do {                            \
+        RCHECK(kt_pre((c), (id)));  \
+        HCHECK((c), (call));        \
+        RCHECK(kt_post((c), (id))); \
+    } while (0)
+
 // exp(s * dt * nnt * A) for the periodic nl x nl square lattice, from its plane-wave
 // eigenbasis: the adjacency (src/square_lattice.cpp:48-61) has eigenvalues
 // 2 cos kx + 2 cos ky with multiplicities counted like the accumulated entries.
@@ -150,6 +179,23 @@ void lattice_expk(int nl, double x, std::vector<double>& out)
         for (int i = 0; i < N; ++i) {
             const int dx = ((j % nl) - (i % nl) + nl) % nl, dy = ((j / nl) - (i / nl) + nl) % nl;
             out[i + (size_t)j * N] = f[dx + nl * dy];
+        }
+}
+
+// exp(x A1) for the periodic ring of nl sites (A1[i][i+1] += 1, A1[i+1][i] += 1, so nl = 2 gives
+// 2 and nl = 1 gives 2 on the diagonal), padded to a 16 x 16 row-major block.  The square
+// lattice's exp(x A) is its Kronecker square.
+void ring_expk16(int nl, double x, std::vector<double>& out)
+{
+    out.assign(256, 0.0);
+    for (int i = 0; i < nl; ++i)
+        for (int j = 0; j < nl; ++j) {
+            double s = 0.0;
+            for (int m = 0; m < nl; ++m) {
+                const double k = 2.0 * M_PI * m / nl;
+                s += std::cos(k * (i - j)) * std::exp(2.0 * x * std::cos(k));
+            }
+            out[i * 16 + j] = s / nl;
         }
 }
 
@@ -181,7 +227,32 @@ int gemm(dqmc_ctx* c, const T* A, long long sA, int lda, int tA, const T* B, lon
     g.alpha = alpha; g.beta = beta;
     g.rowd = rowd; g.innd = innd; g.cold = cold;
     g.batch = c->B;
-    HCHECK(c, launch_gemm<T>(g, sizeof(T) == 8 ? c->tile64 : c->tile32, c->stream));
+    TIMED(c, KT_GEMM, launch_gemm<T>(g, sizeof(T) == 8 ? c->tile64 : c->tile32, c->stream));
+    return 0;
+}
+
+hipError_t qr_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int batch, hipStream_t s)
+{
+    return c->experimental ? launch_qr(buf, stride, N, W, batch, s) : launch_qr_v1(buf, stride, N, W, batch, s);
+}
+hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int c0, int P, int batch, hipStream_t s)
+{
+    return c->experimental ? launch_trsm_upper(buf, stride, N, W, c0, P, batch, s)
+                           : launch_trsm_upper_v1(buf, stride, N, W, c0, P, batch, s);
+}
+
+// C = diag(r) (E(x)E) diag(s) X diag(cv)  (left) or  diag(r) X diag(s) (E(x)E) diag(cv)  (right)
+template <typename T>
+int kron(dqmc_ctx* c, bool left, const T* X, T* C, const T* E, DiagArg rowd = DiagArg(), DiagArg innd = DiagArg(),
+         DiagArg cold = DiagArg())
+{
+    KronArgs<T> k;
+    k.X = X; k.sX = c->NN(); k.C = C; k.sC = c->NN(); k.E = E;
+    k.nl = c->nl; k.N = c->N;
+    k.rowd = rowd; k.innd = innd; k.cold = cold;
+    k.batch = c->B;
+    if (left) TIMED(c, KT_KRON, launch_kron_left<T>(k, c->stream));
+    else TIMED(c, KT_KRON, launch_kron_right<T>(k, c->stream));
     return 0;
 }
 
@@ -190,14 +261,26 @@ int gemm(dqmc_ctx* c, const T* A, long long sA, int lda, int tA, const T* B, lon
 int wrap_up(dqmc_ctx* c, int t)
 {
     const long long nn = c->NN();
+    const int N = c->N;
+    if (c->experimental) {
+        // D (E(x)E) G (F(x)F) D^{-1}: the chemical potential sits in the D tables
+        if (c->prec == DQMC_MIXED) {
+            RCHECK(kron<float>(c, true, c->G32, c->T32, c->E1f, dfield(c, c->tab, t)));
+            RCHECK(kron<float>(c, false, c->T32, c->G32, c->F1f, dnone(), dnone(), dfield(c, c->itab, t)));
+        } else {
+            RCHECK(kron<double>(c, true, c->G64, c->W1, c->E1, dfield(c, c->tab, t)));
+            RCHECK(kron<double>(c, false, c->W1, c->G64, c->F1, dnone(), dnone(), dfield(c, c->itab, t)));
+        }
+        return 0;
+    }
     if (c->prec == DQMC_MIXED) {
-        RCHECK(gemm<float>(c, c->expK32, 0, c->N, 0, c->G32, nn, c->N, 0, c->T32, nn, c->N, 0));
-        RCHECK(gemm<float>(c, c->T32, nn, c->N, 0, c->invExpK32, 0, c->N, 0, c->G32, nn, c->N, 0,
-                           dfield(c, c->tab, t), dnone(), dfield(c, c->itab, t)));
+        RCHECK(gemm<float>(c, c->expK32, 0, N, 0, c->G32, nn, N, 0, c->T32, nn, N, 0));
+        RCHECK(gemm<float>(c, c->T32, nn, N, 0, c->invExpK32, 0, N, 0, c->G32, nn, N, 0, dfield(c, c->tab, t), dnone(),
+                           dfield(c, c->itab, t)));
     } else {
-        RCHECK(gemm<double>(c, c->expK, 0, c->N, 0, c->G64, nn, c->N, 0, c->W1, nn, c->N, 0));
-        RCHECK(gemm<double>(c, c->W1, nn, c->N, 0, c->invExpK, 0, c->N, 0, c->G64, nn, c->N, 0,
-                            dfield(c, c->tab, t), dnone(), dfield(c, c->itab, t)));
+        RCHECK(gemm<double>(c, c->expK, 0, N, 0, c->G64, nn, N, 0, c->W1, nn, N, 0));
+        RCHECK(gemm<double>(c, c->W1, nn, N, 0, c->invExpK, 0, N, 0, c->G64, nn, N, 0, dfield(c, c->tab, t), dnone(),
+                            dfield(c, c->itab, t)));
     }
     return 0;
 }
@@ -205,29 +288,51 @@ int wrap_up(dqmc_ctx* c, int t)
 int wrap_down(dqmc_ctx* c, int t)
 {
     const long long nn = c->NN();
+    const int N = c->N;
+    if (c->experimental) {
+        // (F(x)F) D^{-1} G D (E(x)E)
+        if (c->prec == DQMC_MIXED) {
+            RCHECK(kron<float>(c, false, c->G32, c->T32, c->E1f, dnone(), dfield(c, c->tab, t)));
+            RCHECK(kron<float>(c, true, c->T32, c->G32, c->F1f, dnone(), dfield(c, c->itab, t)));
+        } else {
+            RCHECK(kron<double>(c, false, c->G64, c->W1, c->E1, dnone(), dfield(c, c->tab, t)));
+            RCHECK(kron<double>(c, true, c->W1, c->G64, c->F1, dnone(), dfield(c, c->itab, t)));
+        }
+        return 0;
+    }
     if (c->prec == DQMC_MIXED) {
-        RCHECK(gemm<float>(c, c->G32, nn, c->N, 0, c->expK32, 0, c->N, 0, c->T32, nn, c->N, 0, dnone(),
-                           dfield(c, c->tab, t)));
-        RCHECK(gemm<float>(c, c->invExpK32, 0, c->N, 0, c->T32, nn, c->N, 0, c->G32, nn, c->N, 0, dnone(),
-                           dfield(c, c->itab, t)));
+        RCHECK(gemm<float>(c, c->G32, nn, N, 0, c->expK32, 0, N, 0, c->T32, nn, N, 0, dnone(), dfield(c, c->tab, t)));
+        RCHECK(gemm<float>(c, c->invExpK32, 0, N, 0, c->T32, nn, N, 0, c->G32, nn, N, 0, dnone(), dfield(c, c->itab, t)));
     } else {
-        RCHECK(gemm<double>(c, c->G64, nn, c->N, 0, c->expK, 0, c->N, 0, c->W1, nn, c->N, 0, dnone(),
-                            dfield(c, c->tab, t)));
-        RCHECK(gemm<double>(c, c->invExpK, 0, c->N, 0, c->W1, nn, c->N, 0, c->G64, nn, c->N, 0, dnone(),
-                            dfield(c, c->itab, t)));
+        RCHECK(gemm<double>(c, c->G64, nn, N, 0, c->expK, 0, N, 0, c->W1, nn, N, 0, dnone(), dfield(c, c->tab, t)));
+        RCHECK(gemm<double>(c, c->invExpK, 0, N, 0, c->W1, nn, N, 0, c->G64, nn, N, 0, dnone(), dfield(c, c->itab, t)));
     }
     return 0;
+}
+// X_out = B(t) X (rowd = D) or B(t)^T X (innd = D) with one shared exp(-dt K)
+int left_expk(dqmc_ctx* c, const double* X, double* C, const double* table, int t, bool inner)
+{
+    const long long nn = c->NN();
+    const DiagArg d = dfield(c, table, t);
+    if (c->experimental) return kron<double>(c, true, X, C, table == c->tab ? c->E1 : c->F1, inner ? dnone() : d, inner ? d : dnone());
+    return gemm<double>(c, table == c->tab ? c->expK : c->invExpK, 0, c->N, 0, X, nn, c->N, 0, C, nn, c->N, 0,
+                        inner ? dnone() : d, inner ? d : dnone());
+}
+// X_out = X exp(+-dt K) diag(d) (col scaling) or X diag(d) exp(+-dt K) (inner scaling)
+int right_expk(dqmc_ctx* c, const double* X, double* C, const double* table, int t, bool inner)
+{
+    const long long nn = c->NN();
+    const DiagArg d = dfield(c, table, t);
+    const bool fwd = table == c->tab;   // tab goes with exp(-dt K), itab with exp(+dt K)
+    if (c->experimental)
+        return kron<double>(c, false, X, C, fwd ? c->E1 : c->F1, dnone(), inner ? d : dnone(), inner ? dnone() : d);
+    return gemm<double>(c, X, nn, c->N, 0, fwd ? c->expK : c->invExpK, 0, c->N, 0, C, nn, c->N, 0, dnone(),
+                        inner ? d : dnone(), inner ? dnone() : d);
 }
 // stacks <- B(t) stacks (src/dqmc_core.cpp:190) or B(t)^T stacks (:251, adjB, src/hubbard.cpp:333-341)
 int chain_mul(dqmc_ctx* c, int t, bool adjoint)
 {
-    const long long nn = c->NN();
-    if (!adjoint)
-        RCHECK(gemm<double>(c, c->expK, 0, c->N, 0, c->stk, nn, c->N, 0, c->stk_alt, nn, c->N, 0,
-                            dfield(c, c->tab, t)));
-    else
-        RCHECK(gemm<double>(c, c->expK, 0, c->N, 0, c->stk, nn, c->N, 0, c->stk_alt, nn, c->N, 0, dnone(),
-                            dfield(c, c->tab, t)));
+    RCHECK(left_expk(c, c->stk, c->stk_alt, c->tab, t, adjoint));
     std::swap(c->stk, c->stk_alt);
     return 0;
 }
@@ -239,14 +344,14 @@ int metropolis(dqmc_ctx* c, int t)
         a.fields = c->fields; a.N = c->N; a.L = c->L; a.t = t;
         a.rng = c->rng; a.powtab = c->powtab; a.delta = c->delta;
         a.accmask = c->accmask; a.acc_count = c->acc_count; a.kmax = c->kmax;
-        HCHECK(c, launch_metropolis<float>(a, c->B, c->stream));
+        TIMED(c, KT_METRO, launch_metropolis<float>(a, c->B, c->stream));
     } else {
         MetroArgs<double> a;
         a.G = c->G64; a.sG = c->NN();
         a.fields = c->fields; a.N = c->N; a.L = c->L; a.t = t;
         a.rng = c->rng; a.powtab = c->powtab; a.delta = c->delta;
         a.accmask = c->accmask; a.acc_count = c->acc_count; a.kmax = c->kmax;
-        HCHECK(c, launch_metropolis<double>(a, c->B, c->stream));
+        TIMED(c, KT_METRO, launch_metropolis<double>(a, c->B, c->stream));
     }
     return 0;
 }
@@ -269,7 +374,7 @@ int stack_push(dqmc_ctx* c, DStack& s)
         M = c->W1;
     }
     HCHECK(c, launch_udv_prepare(M, nn, c->qr, c->sqr(), c->perm, N, c->B, c->stream));
-    HCHECK(c, launch_qr(c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
+    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
     HCHECK(c, launch_udv_finish(c->qr, c->sqr(), c->perm, sU(c, s, s.len), st, sD(c, s, s.len), sd, c->W2, nn, N,
                                 c->B, c->stream));
     if (s.len == 0)
@@ -287,8 +392,8 @@ int solve_00bb(dqmc_ctx* c, const double* u, const double* v, const double* d, d
     const long long nn = c->NN(), st = (long long)c->S * nn, sd = (long long)c->S * c->N;
     const int N = c->N;
     HCHECK(c, launch_build_00bb(u, st, v, st, d, sd, c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
-    HCHECK(c, launch_qr(c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
-    HCHECK(c, launch_trsm_upper(c->qr, c->sqr(), N, 2 * N, N, N, c->B, c->stream));
+    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
+    TIMED(c, KT_TRSM, trsm_any(c, c->qr, c->sqr(), N, 2 * N, N, N, c->B, c->stream));
     HCHECK(c, launch_extract(c->qr, c->sqr(), 2 * N, N, out, nn, N, transposed_result ? 0 : 1, c->B, c->stream));
     return 0;
 }
@@ -312,8 +417,8 @@ int recompute_eq(dqmc_ctx* c, double* out)
     RCHECK(gemm<double>(c, Vl, st, N, 1, Vr, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_MIN1, Dl, sd), dnone(),
                         dvec(DIAG_MIN1, Dr, sd), 1.0, 1.0));
     HCHECK(c, launch_build_rhs(Ul, st, dvec(DIAG_INVMAX1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, N, c->B, c->stream));
-    HCHECK(c, launch_qr(c->qr, c->sqr(), N, W, c->B, c->stream));
-    HCHECK(c, launch_trsm_upper(c->qr, c->sqr(), N, W, N, N, c->B, c->stream));
+    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, W, c->B, c->stream));
+    TIMED(c, KT_TRSM, trsm_any(c, c->qr, c->sqr(), N, W, N, N, c->B, c->stream));
     // G = Ur Drmax^{-1} X
     RCHECK(gemm<double>(c, Ur, st, N, 0, c->qr + N, c->sqr(), W, 1, out, nn, N, 0, dnone(), dvec(DIAG_INVMAX1, Dr, sd)));
     return 0;
@@ -346,8 +451,8 @@ int recompute_dyn(dqmc_ctx* c, double* gtt, double* gt0n, double* g0tn)
                         dvec(DIAG_MIN1, Dr, sd), 1.0, 1.0));
     HCHECK(c, launch_build_rhs(Ul, st, dvec(DIAG_INVMAX1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, N, c->B, c->stream));
     HCHECK(c, launch_build_rhs(Vl, st, dvec(DIAG_MIN1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, 2 * N, c->B, c->stream));
-    HCHECK(c, launch_qr(c->qr, c->sqr(), N, W, c->B, c->stream));
-    HCHECK(c, launch_trsm_upper(c->qr, c->sqr(), N, W, N, 2 * N, c->B, c->stream));
+    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, W, c->B, c->stream));
+    TIMED(c, KT_TRSM, trsm_any(c, c->qr, c->sqr(), N, W, N, 2 * N, c->B, c->stream));
     RCHECK(gemm<double>(c, Ur, st, N, 0, c->qr + N, c->sqr(), W, 1, gtt, nn, N, 0, dnone(), dvec(DIAG_INVMAX1, Dr, sd)));
     RCHECK(gemm<double>(c, Ur, st, N, 0, c->qr + 2 * N, c->sqr(), W, 1, gt0n, nn, N, 0, dnone(),
                         dvec(DIAG_INVMAX1, Dr, sd)));
@@ -440,9 +545,8 @@ int do_sweep_dynamic(dqmc_ctx* c)
         RCHECK(wrap_up(c, ti));
         double* gt0n = (c->gt0 == c->gt0A) ? c->gt0B : c->gt0A;
         double* g0tn = (c->g0t == c->g0tA) ? c->g0tB : c->g0tA;
-        RCHECK(gemm<double>(c, c->expK, 0, N, 0, c->gt0, nn, N, 0, gt0n, nn, N, 0, dfield(c, c->tab, ti)));
-        RCHECK(gemm<double>(c, c->g0t, nn, N, 0, c->invExpK, 0, N, 0, g0tn, nn, N, 0, dnone(), dnone(),
-                            dfield(c, c->itab, ti)));
+        RCHECK(left_expk(c, c->gt0, gt0n, c->tab, ti, false));
+        RCHECK(right_expk(c, c->g0t, g0tn, c->itab, ti, false));
         c->gt0 = gt0n; c->g0t = g0tn;
         RCHECK(chain_mul(c, ti, false));
         if (t % c->pace == 0 || t == L) {
@@ -473,7 +577,7 @@ int do_sweep_dynamic(dqmc_ctx* c)
 typedef int (*sweep_fn)(dqmc_ctx*);
 int run_sweep(dqmc_ctx* c, sweep_fn fn, int kind)
 {
-    if (!c->use_graphs) {
+    if (!c->use_graphs || c->kt_kernel >= 0) {
         RCHECK(fn(c));
         HCHECK(c, hipStreamSynchronize(c->stream));
         return 0;
@@ -551,11 +655,10 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     if (p->nl < 1 || p->nt < 1 || p->stabilization_pace < 1 || p->u < 0.0 || p->beta < 0.0) return DQMC_EINVAL;
     if (precision != DQMC_FP64 && precision != DQMC_MIXED) return DQMC_EINVAL;
     const int N = p->nl * p->nl;
-    if (N > 256) return DQMC_EINVAL;
-    if (delay_k < 1 || delay_k > 64) return DQMC_EINVAL;
+    if (p->nl > 16) return DQMC_EINVAL;   // N <= 256: Kronecker tiles and QR panels
+    if (delay_k < 1 || delay_k > 32) return DQMC_EINVAL;
     const size_t esz = precision == DQMC_MIXED ? 4 : 8;
-    const int kalloc = delay_k == 1 ? 1 : ((delay_k + 7) / 8) * 8;
-    if (metropolis_lds_bytes(N, kalloc, esz) > 160 * 1024) return DQMC_EINVAL;
+    if (metropolis_lds_bytes(N, delay_k, esz) > 160 * 1024) return DQMC_EINVAL;
 
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return DQMC_ENODEV;
@@ -567,6 +670,10 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     c->beta = p->beta; c->dt = p->beta / p->nt; c->nnt = p->nnt; c->u = p->u; c->mu = p->mu;
     c->alpha = std::acosh(std::exp(0.5 * c->dt * c->u));   // src/hubbard.cpp:35
     c->eqtime = p->collect_eqtime != 0; c->dynamic = p->collect_dynamic != 0;
+    {
+        const char* ex = std::getenv("DQMC_EXPERIMENTAL_KERNELS");
+        c->experimental = ex && ex[0] == '1';
+    }
     c->chain_mu.assign(n_chains, p->mu);
     c->rng_host.assign(n_chains, 1ULL);
     *out = c;
@@ -574,9 +681,11 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     const int B = n_chains, L = c->L, S = c->S;
     const long long nn = c->NN();
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return c->fail(DQMC_EHIP, "stream");
-    HCHECK(c, prepare_kernels());
+    HCHECK(c, prepare_kernels(c->experimental));
     for (auto& e : c->ev) HCHECK(c, hipEventCreate(&e));
     RCHECK(c->alloc(&c->expK, nn * 8)); RCHECK(c->alloc(&c->invExpK, nn * 8));
+    RCHECK(c->alloc(&c->E1, 256 * 8)); RCHECK(c->alloc(&c->F1, 256 * 8));
+    RCHECK(c->alloc(&c->E1f, 256 * 4)); RCHECK(c->alloc(&c->F1f, 256 * 4));
     RCHECK(c->alloc(&c->tab, (size_t)B * 16)); RCHECK(c->alloc(&c->itab, (size_t)B * 16));
     RCHECK(c->alloc(&c->delta, (size_t)B * 16));
     RCHECK(c->alloc(&c->powtab, (size_t)2 * N * 8)); RCHECK(c->alloc(&c->rng, (size_t)B * 8));
@@ -620,6 +729,17 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         for (long long i = 0; i < nn; ++i) f[i] = (float)iek[i];
         HCHECK(c, hipMemcpy(c->invExpK32, f.data(), nn * 4, hipMemcpyHostToDevice));
     }
+    {
+        std::vector<double> e1, f1;
+        ring_expk16(c->nl, c->dt * c->nnt, e1);
+        ring_expk16(c->nl, -c->dt * c->nnt, f1);
+        std::vector<float> e1f(256), f1f(256);
+        for (int i = 0; i < 256; ++i) { e1f[i] = (float)e1[i]; f1f[i] = (float)f1[i]; }
+        HCHECK(c, hipMemcpy(c->E1, e1.data(), 256 * 8, hipMemcpyHostToDevice));
+        HCHECK(c, hipMemcpy(c->F1, f1.data(), 256 * 8, hipMemcpyHostToDevice));
+        HCHECK(c, hipMemcpy(c->E1f, e1f.data(), 256 * 4, hipMemcpyHostToDevice));
+        HCHECK(c, hipMemcpy(c->F1f, f1f.data(), 256 * 4, hipMemcpyHostToDevice));
+    }
     std::vector<uint64_t> pw(2 * N);
     uint64_t x = 1;
     for (int k = 0; k < 2 * N; ++k) { x = (x * 16807ULL) % kMinstdM; pw[k] = x; }
@@ -640,6 +760,7 @@ void dqmc_destroy(dqmc_ctx* c)
     for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
     for (void* p : c->allocs) (void)hipFree(p);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->kt_ev) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -758,26 +879,14 @@ int dqmc_apply_bop(dqmc_ctx* c, int chain, int op, int t, double* inout)
     if (!inout || op < 0 || op > 4 || t < 0 || t >= c->L) return c->fail(DQMC_EINVAL, "bad apply_bop arguments");
     if (!c->initialized) return c->fail(DQMC_ESTATE, "apply_bop before init");
     const long long nn = c->NN();
-    const int N = c->N;
     // every chain computes on W2 (only `chain`'s result is read back)
     HCHECK(c, hipMemcpy(c->W2 + chain * nn, inout, nn * 8, hipMemcpyHostToDevice));
     switch (op) {
-        case 0:  // B G = D (expK G)
-            RCHECK(gemm<double>(c, c->expK, 0, N, 0, c->W2, nn, N, 0, c->W3, nn, N, 0, dfield(c, c->tab, t)));
-            break;
-        case 1:  // G B = (G D) expK
-            RCHECK(gemm<double>(c, c->W2, nn, N, 0, c->expK, 0, N, 0, c->W3, nn, N, 0, dnone(), dfield(c, c->tab, t)));
-            break;
-        case 2:  // B^{-1} G = invExpK (D^{-1} G)
-            RCHECK(gemm<double>(c, c->invExpK, 0, N, 0, c->W2, nn, N, 0, c->W3, nn, N, 0, dnone(), dfield(c, c->itab, t)));
-            break;
-        case 3:  // G B^{-1} = (G invExpK) D^{-1}
-            RCHECK(gemm<double>(c, c->W2, nn, N, 0, c->invExpK, 0, N, 0, c->W3, nn, N, 0, dnone(), dnone(),
-                                dfield(c, c->itab, t)));
-            break;
-        default:  // B^T G = expK (D G)
-            RCHECK(gemm<double>(c, c->expK, 0, N, 0, c->W2, nn, N, 0, c->W3, nn, N, 0, dnone(), dfield(c, c->tab, t)));
-            break;
+        case 0: RCHECK(left_expk(c, c->W2, c->W3, c->tab, t, false)); break;    // D e^{-dtK} G
+        case 1: RCHECK(right_expk(c, c->W2, c->W3, c->tab, t, true)); break;    // G D e^{-dtK}
+        case 2: RCHECK(left_expk(c, c->W2, c->W3, c->itab, t, true)); break;    // e^{dtK} D^{-1} G
+        case 3: RCHECK(right_expk(c, c->W2, c->W3, c->itab, t, false)); break;  // G e^{dtK} D^{-1}
+        default: RCHECK(left_expk(c, c->W2, c->W3, c->tab, t, true)); break;    // e^{-dtK} D G
     }
     HCHECK(c, hipStreamSynchronize(c->stream));
     HCHECK(c, hipMemcpy(inout, c->W3 + chain * nn, nn * 8, hipMemcpyDeviceToHost));
@@ -906,6 +1015,36 @@ int dqmc_event_elapsed(dqmc_ctx* c, int a, int b, float* ms)
     if (!c || a < 0 || a >= 8 || b < 0 || b >= 8 || !ms) return DQMC_EINVAL;
     HCHECK(c, hipEventSynchronize(c->ev[b]));
     HCHECK(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+    return 0;
+}
+
+int dqmc_kernel_timer_begin(dqmc_ctx* c, int kernel, int max_launches)
+{
+    if (!c || kernel < 0 || kernel > 4 || max_launches < 1) return DQMC_EINVAL;
+    while ((int)c->kt_ev.size() < 2 * max_launches) {
+        hipEvent_t e;
+        HCHECK(c, hipEventCreate(&e));
+        c->kt_ev.push_back(e);
+    }
+    c->kt_kernel = kernel;
+    c->kt_used = 0;
+    return 0;
+}
+
+int dqmc_kernel_timer_end(dqmc_ctx* c, double* total_ms, int* launches)
+{
+    if (!c) return DQMC_EINVAL;
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (int i = 0; i + 1 < c->kt_used; i += 2) {
+        float ms = 0.f;
+        HCHECK(c, hipEventElapsedTime(&ms, c->kt_ev[i], c->kt_ev[i + 1]));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = c->kt_used / 2;
+    c->kt_kernel = -1;
+    c->kt_used = 0;
     return 0;
 }
 

@@ -41,6 +41,25 @@ struct GemmArgs {
     int batch;
 };
 
+// Kronecker-factored hopping exponential.  The periodic nl x nl square lattice has
+// A = A1 (x) I + I (x) A1 (A1 the ring adjacency), so exp(c A) = E (x) E with E = exp(c A1).
+// Left:  C = diag(r) (E(x)E) diag(s) X diag(cv)   (acts on the columns of X)
+// Right: C = diag(r) X diag(s) (E(x)E) diag(cv)   (acts on the rows of X)
+// E is stored padded to 16 x 16 (zeros beyond nl); X and C are N x N column-major, C != X.
+template <typename T>
+struct KronArgs {
+    const T* X; long long sX;
+    T* C; long long sC;
+    const T* E;          // 16 x 16, row-major, padded
+    int nl, N;
+    DiagArg rowd, innd, cold;
+    int batch;
+};
+template <typename T>
+hipError_t launch_kron_left(const KronArgs<T>& a, hipStream_t s);
+template <typename T>
+hipError_t launch_kron_right(const KronArgs<T>& a, hipStream_t s);
+
 // Tile shapes the autotuner chooses from.
 enum GemmTile : int { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_COUNT = 3 };
 
@@ -65,12 +84,17 @@ hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, hipStream_t s);
 // LDS bytes of one Metropolis workgroup (krows pending-update rows of X and Y).
 size_t metropolis_lds_bytes(int N, int krows, size_t esz);
 // One-time kernel attribute setup (large dynamic LDS); call before any graph capture.
-hipError_t prepare_kernels();
+hipError_t prepare_kernels(bool blocked_qr);
 
 // Householder QR (no pivoting) of the leading N x N block of a row-major N x W buffer,
 // applying Q^T to the trailing W-N columns.  On exit the leading block holds R (zeros
 // below the diagonal).
 hipError_t launch_qr(double* buf, long long stride, int N, int W, int batch, hipStream_t s);
+
+// Same contracts, reference-path implementations (see the kernel file).
+hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, hipStream_t s);
+hipError_t launch_trsm_upper_v1(double* buf, long long stride, int N, int W, int c0, int P, int batch,
+                                hipStream_t s);
 
 // In-place X = R^{-1} Z, R the upper triangle of the leading N x N block, Z columns
 // [c0, c0+P) of the same row-major N x W buffer.

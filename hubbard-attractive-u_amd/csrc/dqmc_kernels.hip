@@ -164,6 +164,156 @@ template hipError_t launch_gemm<double>(const GemmArgs<double>&, int, hipStream_
 template hipError_t launch_gemm<float>(const GemmArgs<float>&, int, hipStream_t);
 
 // ---------------------------------------------------------------------------------------------
+//  Kronecker-factored hopping exponential on MFMA.
+//  A vector v of the lattice (site i = x + nl*y) is an nl x nl tile V(x, y).  (E(x)E) v is the
+//  tile E V E^T = E V E (E symmetric).  With 16x16x4 MFMA fragments:
+//    step 1  Y^T = V^T E^T : A-fragment = V^T read from memory, B-fragment = E (constant);
+//    step 2  Z   = Y E     : the A-fragment of step k is exactly item k of step 1's accumulator
+//  (the f64 accumulator layout puts Y^T(4k + l/16, l%16) in lane l, item k), so the tile never
+//  leaves registers.  8 MFMAs per lattice vector.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ int kron_row(int lane, int r) { return Mfma<T>::row(lane, r); }
+
+// Transform one tile: vin(x', y') supplied by `ld(x', y')`, result passed to `st(x, y, value)`.
+template <typename T, typename LD, typename ST>
+__device__ __forceinline__ void kron_tile(const T (&ef)[4], const T (&ef2)[4], int lane, int nl, LD ld, ST st)
+{
+    typedef typename Mfma<T>::acc_t acc_t;
+    acc_t y = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int xp = (lane >> 4) + 4 * s, yp = lane & 15;   // A(row = y', k = x') = V(x', y')
+        const T v = (xp < nl && yp < nl) ? ld(xp, yp) : T(0);
+        y = Mfma<T>::mma(v, ef[s], y);                          // B(k = x', col = x) = E[x][x']
+    }
+    acc_t z = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) z = Mfma<T>::mma(y[s], ef2[s], z);   // A(row = x, k = y') = Y(x, y')
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int x = kron_row<T>(lane, r), yy = lane & 15;
+        if (x < nl && yy < nl) st(x, yy, z[r]);
+    }
+}
+
+// E fragments B(k, col) = E[col][k], col = l%16.  Step 1 contracts k = l/16 + 4s.  Step 2
+// contracts over the rows of step 1's accumulator, item s of lane l: k = Mfma<T>::row(l, s),
+// i.e. 4s + l/16 for f64 and 4(l/16) + s for f32.
+template <typename T>
+__device__ __forceinline__ void load_efrag(const T* E, int lane, T (&ef)[4], T (&ef2)[4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        ef[s] = E[(lane & 15) * 16 + (lane >> 4) + 4 * s];
+        ef2[s] = E[(lane & 15) * 16 + Mfma<T>::row(lane, s)];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kron_left_kernel(KronArgs<T> a)
+{
+    const int b = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nl = a.nl, N = a.N;
+    T ef[4], ef2[4];
+    load_efrag<T>(a.E, lane, ef, ef2);
+    const T* X = a.X + (long long)b * a.sX;
+    T* C = a.C + (long long)b * a.sC;
+    for (int cc = 0; cc < 4; ++cc) {
+        const int c = blockIdx.x * 16 + wave * 4 + cc;
+        if (c >= N) break;
+        const double cs = (a.cold.mode != DIAG_NONE) ? diag_value(a.cold, b, c) : 1.0;
+        const T* xc = X + (long long)c * N;
+        T* cc_out = C + (long long)c * N;
+        kron_tile<T>(ef, ef2, lane, nl,
+            [&](int xp, int yp) {
+                const int i = xp + nl * yp;
+                T v = xc[i];
+                if (a.innd.mode != DIAG_NONE) v = T((double)v * diag_value(a.innd, b, i));
+                return v;
+            },
+            [&](int x, int y, T z) {
+                const int i = x + nl * y;
+                double v = (double)z * cs;
+                if (a.rowd.mode != DIAG_NONE) v *= diag_value(a.rowd, b, i);
+                cc_out[i] = T(v);
+            });
+    }
+}
+
+// Right multiplication: 16 consecutive rows per workgroup are staged through LDS (coalesced
+// column segments), transformed per row, and written back.
+template <typename T>
+__global__ __launch_bounds__(256) void kron_right_kernel(KronArgs<T> a)
+{
+    __shared__ T tile[256][17];
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nl = a.nl, N = a.N;
+    const int r0 = blockIdx.x * 16;
+    const T* X = a.X + (long long)b * a.sX;
+    T* C = a.C + (long long)b * a.sC;
+    for (int e = tid; e < N * 16; e += 256) {
+        const int rr = e & 15, c = e >> 4;
+        const int row = r0 + rr;
+        T v = T(0);
+        if (row < N) {
+            v = X[row + (long long)c * N];
+            if (a.innd.mode != DIAG_NONE) v = T((double)v * diag_value(a.innd, b, c));
+        }
+        tile[c][rr] = v;
+    }
+    __syncthreads();
+    T ef[4], ef2[4];
+    load_efrag<T>(a.E, lane, ef, ef2);
+    for (int rr = wave; rr < 16; rr += 4) {
+        T vin[4];
+        // read the whole row first: the tile is transformed in place
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int xp = (lane >> 4) + 4 * s, yp = lane & 15;
+            vin[s] = (xp < nl && yp < nl) ? tile[xp + nl * yp][rr] : T(0);
+        }
+        kron_tile<T>(ef, ef2, lane, nl,
+            [&](int xp, int yp) {
+                return vin[(xp - (lane >> 4)) >> 2];
+            },
+            [&](int x, int y, T z) { tile[x + nl * y][rr] = z; });
+    }
+    __syncthreads();
+    for (int e = tid; e < N * 16; e += 256) {
+        const int rr = e & 15, c = e >> 4;
+        const int row = r0 + rr;
+        if (row < N) {
+            double v = (double)tile[c][rr];
+            if (a.rowd.mode != DIAG_NONE) v *= diag_value(a.rowd, b, row);
+            if (a.cold.mode != DIAG_NONE) v *= diag_value(a.cold, b, c);
+            C[row + (long long)c * N] = T(v);
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_kron_left(const KronArgs<T>& a, hipStream_t s)
+{
+    if (a.nl > 16) return hipErrorInvalidValue;
+    dim3 grid((a.N + 15) / 16, a.batch);
+    hipLaunchKernelGGL(kron_left_kernel<T>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_kron_right(const KronArgs<T>& a, hipStream_t s)
+{
+    if (a.nl > 16) return hipErrorInvalidValue;
+    dim3 grid((a.N + 15) / 16, a.batch);
+    hipLaunchKernelGGL(kron_right_kernel<T>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template hipError_t launch_kron_left<double>(const KronArgs<double>&, hipStream_t);
+template hipError_t launch_kron_right<double>(const KronArgs<double>&, hipStream_t);
+template hipError_t launch_kron_left<float>(const KronArgs<float>&, hipStream_t);
+template hipError_t launch_kron_right<float>(const KronArgs<float>&, hipStream_t);
+
+// ---------------------------------------------------------------------------------------------
 //  Metropolis with delayed updates.
 //
 //  For site i of slice t (sites in order 0..N-1, src/dqmc_core.cpp:106):
@@ -189,49 +339,6 @@ __device__ __forceinline__ double canonical_from_draws(uint64_t x1, uint64_t x2)
     return u;
 }
 
-template <typename T, int KK>
-__device__ __forceinline__ void flush_rows(T* __restrict__ G, int N, const T* __restrict__ X,
-                                           const T* __restrict__ Y, double* diag0, int tid, int nth)
-{
-    for (int j = tid; j < N; j += nth) {
-        T xr[KK];
-#pragma unroll
-        for (int m = 0; m < KK; ++m) xr[m] = X[m * N + j];
-        for (int c = 0; c < N; ++c) {
-            T acc = T(0);
-#pragma unroll
-            for (int m = 0; m < KK; ++m) acc += xr[m] * Y[m * N + c];
-            const T gn = G[j + (long long)c * N] + acc;
-            G[j + (long long)c * N] = gn;
-            if (c == j) diag0[j] = (double)gn;
-        }
-    }
-}
-
-template <typename T>
-__device__ void flush_pending(T* G, int N, T* X, T* Y, double* diag0, double* dcorr, int kc, int kmax, int tid,
-                              int nth)
-{
-    // pad the pending terms with zeros up to the unrolled width
-    const int kk = kmax == 1 ? 1 : ((kc + 7) / 8) * 8;
-    for (int m = kc; m < kk; ++m)
-        for (int j = tid; j < N; j += nth) { X[m * N + j] = T(0); Y[m * N + j] = T(0); }
-    __syncthreads();
-    switch (kk) {
-        case 1: flush_rows<T, 1>(G, N, X, Y, diag0, tid, nth); break;
-        case 8: flush_rows<T, 8>(G, N, X, Y, diag0, tid, nth); break;
-        case 16: flush_rows<T, 16>(G, N, X, Y, diag0, tid, nth); break;
-        case 24: flush_rows<T, 24>(G, N, X, Y, diag0, tid, nth); break;
-        case 32: flush_rows<T, 32>(G, N, X, Y, diag0, tid, nth); break;
-        case 40: flush_rows<T, 40>(G, N, X, Y, diag0, tid, nth); break;
-        case 48: flush_rows<T, 48>(G, N, X, Y, diag0, tid, nth); break;
-        case 56: flush_rows<T, 56>(G, N, X, Y, diag0, tid, nth); break;
-        default: flush_rows<T, 64>(G, N, X, Y, diag0, tid, nth); break;
-    }
-    for (int j = tid; j < N; j += nth) dcorr[j] = 0.0;
-    __syncthreads();
-}
-
 __device__ __forceinline__ double accept_prob(double gii, double delta, double* xout)
 {
 #pragma clang fp contract(off)
@@ -241,95 +348,213 @@ __device__ __forceinline__ double accept_prob(double gii, double delta, double* 
     return fmin(1.0, fabs(r));
 }
 
+#define MB_MAX 32   // largest site block (delay) of the submatrix update
+
+// LDS layout of one Metropolis workgroup
+template <typename T>
+struct MetroLds {
+    T* Xl;         // [kb][N]  X^T of the pending rank-a update (rows m, G-row index contiguous)
+    T* Yl;         // [kb][N]  Y (rows m, G-column index contiguous)
+    double* Gs;    // [kb][kb+1] current G restricted to the site block
+    double* XS;    // [kb][kb]  gamma_m * G(S, i_m) of accepted flip m (restricted to the block)
+    double* YS;    // [kb][kb]  G(i_m, S) - e_{i_m} (restricted to the block)
+    double* unif;  // [N]
+    double* gam;   // [kb]
+    int* sl;       // [N] fields of the slice
+    int* ql;       // [kb] block-local index of accepted flip m
+    uint32_t* mask;
+};
+
+template <typename T>
+__device__ MetroLds<T> metro_lds(unsigned char* smem, int N, int kb)
+{
+    MetroLds<T> L;
+    const int kb4 = ((kb + 3) / 4) * 4;
+    L.Xl = reinterpret_cast<T*>(smem);
+    L.Yl = L.Xl + (size_t)kb4 * N;
+    L.Gs = reinterpret_cast<double*>(L.Yl + (size_t)kb4 * N);
+    L.XS = L.Gs + (size_t)kb * (kb + 1);
+    L.YS = L.XS + (size_t)kb * kb;
+    L.unif = L.YS + (size_t)kb * kb;
+    L.gam = L.unif + N;
+    L.sl = reinterpret_cast<int*>(L.gam + kb);
+    L.ql = L.sl + N;
+    L.mask = reinterpret_cast<uint32_t*>(L.ql + kb);
+    return L;
+}
+
+size_t metropolis_lds_bytes(int N, int kb, size_t esz)
+{
+    const int kb4 = ((kb + 3) / 4) * 4;
+    size_t b = (size_t)2 * kb4 * N * esz;
+    b = (b + 7) / 8 * 8;
+    b += (size_t)8 * ((size_t)kb * (kb + 1) + 2 * (size_t)kb * kb + N + kb);
+    b += (size_t)4 * (N + kb) + (size_t)4 * ((N + 31) / 32) + 64;
+    return b;
+}
+
+// G(ri, cj) += sum_m X[ri][m] Y[m][cj] over the whole N x N matrix, on MFMA.  The tile is
+// computed transposed (MFMA rows = G columns) so that each lane group touches contiguous
+// column segments of the column-major G.
+template <typename T>
+__device__ void flush_mfma(T* __restrict__ G, int N, const T* __restrict__ Xl, const T* __restrict__ Yl, int a4)
+{
+    typedef typename Mfma<T>::acc_t acc_t;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nt = (N + 15) / 16;
+    for (int tile = wave; tile < nt * nt; tile += nw) {
+        const int c0 = (tile / nt) * 16, r0 = (tile % nt) * 16;   // G column block, G row block
+        const int gr = r0 + (lane & 15);
+        acc_t acc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gc = c0 + Mfma<T>::row(lane, r);
+            acc[r] = (gr < N && gc < N) ? G[gr + (long long)gc * N] : T(0);
+        }
+        for (int k0 = 0; k0 < a4; k0 += 4) {
+            const int kk = k0 + (lane >> 4);
+            const int ca = c0 + (lane & 15);
+            const T av = (ca < N) ? Yl[kk * N + ca] : T(0);   // A(row = G column, k)
+            const T bv = (gr < N) ? Xl[kk * N + gr] : T(0);   // B(k, col = G row)
+            acc = Mfma<T>::mma(av, bv, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gc = c0 + Mfma<T>::row(lane, r);
+            if (gr < N && gc < N) G[gr + (long long)gc * N] = acc[r];
+        }
+    }
+}
+
+// Submatrix Metropolis update of slice t.  Sites are visited in blocks S of kb consecutive
+// sites.  Inside a block the accept/reject sequence runs on Gs = G(S,S) in LDS with the
+// reference's rank-1 update.  After the block the accepted flips A are applied to the whole
+// G as one rank-|A| product G <- G + X Y, where
+//   X(:,m) = gamma_m (G0(:,i_m) + sum_{m'<m} X(:,m') Y(m',i_m))
+//   Y(m,:) = G0(i_m,:) - e_{i_m} + sum_{m'<m} X(i_m,m') Y(m',:)
+// are the sequential Sherman-Morrison vectors (src/hubbard.cpp:132-139) rebuilt from the
+// columns/rows of G0 = G at block start; the block-restricted entries they need are the ones
+// recorded in XS / YS while the block was processed.  With kb = 1 every accepted flip is
+// applied immediately, as in the reference.
 template <typename T>
 __global__ __launch_bounds__(256) void metropolis_kernel(MetroArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    const int N = a.N, kmax = a.kmax;
+    const int N = a.N, kb = a.kmax;
     const int b = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
-    const int kal = kmax == 1 ? 1 : ((kmax + 7) / 8) * 8;   // rows of X / Y (flushes pad to 8)
-    T* X = reinterpret_cast<T*>(smem);
-    T* Y = X + (size_t)kal * N;
-    double* dcorr = reinterpret_cast<double*>(Y + (size_t)kal * N);
-    double* diag0 = dcorr + N;
-    double* unif = diag0 + N;
-    int* sl = reinterpret_cast<int*>(unif + N);
-    uint32_t* mask = reinterpret_cast<uint32_t*>(sl + N);
+    MetroLds<T> L = metro_lds<T>(smem, N, kb);
 
     T* G = a.G + (long long)b * a.sG;
     int8_t* f = a.fields + ((long long)b * a.L + a.t) * N;
     const uint64_t x0 = a.rng[b];
     const int nw = (N + 31) / 32;
-
     for (int j = tid; j < N; j += nth) {
-        sl[j] = f[j];
-        diag0[j] = (double)G[j + (long long)j * N];
-        dcorr[j] = 0.0;
+        L.sl[j] = f[j];
         const uint64_t x1 = (x0 * a.powtab[2 * j]) % MINSTD_M;
         const uint64_t x2 = (x0 * a.powtab[2 * j + 1]) % MINSTD_M;
-        unif[j] = canonical_from_draws(x1, x2);
+        L.unif[j] = canonical_from_draws(x1, x2);
     }
-    for (int w = tid; w < nw; w += nth) mask[w] = 0u;
-    __syncthreads();
+    for (int w = tid; w < nw; w += nth) L.mask[w] = 0u;
 
     const double dpos = a.delta[2 * b], dneg = a.delta[2 * b + 1];
-    int kc = 0, nacc = 0;
-    for (int i = 0; i < N; ++i) {
-        const double gii = diag0[i] + dcorr[i];
-        const int si = sl[i];
-        const double delta = si > 0 ? dpos : dneg;
-        double x;
-        const double p = accept_prob(gii, delta, &x);
-        if (unif[i] < p) {
-            const double gam = delta / x;
-            for (int j = tid; j < N; j += nth) {
-                T aj = G[j + (long long)i * N];
-                T bj = G[i + (long long)j * N];
-                for (int m = 0; m < kc; ++m) {
-                    aj += X[m * N + j] * Y[m * N + i];
-                    bj += X[m * N + i] * Y[m * N + j];
+    int nacc = 0;
+    for (int b0 = 0; b0 < N; b0 += kb) {
+        const int bs = min(kb, N - b0);
+        __syncthreads();
+        for (int e = tid; e < bs * bs; e += nth) {
+            const int r = e % bs, c = e / bs;
+            L.Gs[r * (kb + 1) + c] = (double)G[(b0 + r) + (long long)(b0 + c) * N];
+        }
+        __syncthreads();
+        int na = 0;
+        for (int q = 0; q < bs; ++q) {
+            const int i = b0 + q;
+            const double gii = L.Gs[q * (kb + 1) + q];
+            const int si = L.sl[i];
+            const double delta = si > 0 ? dpos : dneg;
+            double x;
+            const double p = accept_prob(gii, delta, &x);
+            if (L.unif[i] < p) {
+                const double gam = delta / x;
+                if (tid < bs) {
+                    L.XS[na * kb + tid] = gam * L.Gs[tid * (kb + 1) + q];
+                    L.YS[na * kb + tid] = L.Gs[q * (kb + 1) + tid] - (tid == q ? 1.0 : 0.0);
                 }
-                if (j == i) bj -= T(1);
-                const T xv = T(gam * (double)aj);
-                X[kc * N + j] = xv;
-                Y[kc * N + j] = bj;
-                dcorr[j] += (double)xv * (double)bj;
-            }
-            if (tid == 0) {
-                sl[i] = -si;
-                f[i] = (int8_t)(-si);
-                mask[i >> 5] |= 1u << (i & 31);
-            }
-            ++kc;
-            ++nacc;
-            __syncthreads();
-            if (kc == kmax) {
-                flush_pending<T>(G, N, X, Y, diag0, dcorr, kc, kmax, tid, nth);
-                kc = 0;
+                if (tid == 0) {
+                    L.gam[na] = gam;
+                    L.ql[na] = q;
+                }
+                // every wave has read sl[i] for this site once all are past this barrier
+                __syncthreads();
+                if (tid == 0) {
+                    L.sl[i] = -si;
+                    f[i] = (int8_t)(-si);
+                    L.mask[i >> 5] |= 1u << (i & 31);
+                }
+                for (int e = tid; e < bs * bs; e += nth) {
+                    const int r = e % bs, c = e / bs;
+                    L.Gs[r * (kb + 1) + c] += L.XS[na * kb + r] * L.YS[na * kb + c];
+                }
+                ++na;
+                __syncthreads();
             }
         }
+        nacc += na;
+        if (na == 0) continue;
+        // rebuild X (rows of G) and Y (columns of G) of the rank-na update from G0
+        const int a4 = ((na + 3) / 4) * 4;
+        for (int j = tid; j < N; j += nth) {
+            double xr[MB_MAX], yr[MB_MAX];
+#pragma unroll
+            for (int m = 0; m < MB_MAX; ++m) {
+                if (m < na) {
+                    const int im = b0 + L.ql[m];
+                    xr[m] = (double)G[j + (long long)im * N];
+                    yr[m] = (double)G[im + (long long)j * N] - (j == im ? 1.0 : 0.0);
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < MB_MAX; ++m) {
+                if (m < na) {
+                    const int qm = L.ql[m];
+                    double sx = xr[m], sy = yr[m];
+#pragma unroll
+                    for (int mp = 0; mp < m; ++mp) {
+                        sx += xr[mp] * L.YS[mp * kb + qm];
+                        sy += L.XS[mp * kb + qm] * yr[mp];
+                    }
+                    xr[m] = L.gam[m] * sx;
+                    yr[m] = sy;
+                    L.Xl[m * N + j] = T(xr[m]);
+                    L.Yl[m * N + j] = T(yr[m]);
+                }
+            }
+            for (int m = na; m < a4; ++m) { L.Xl[m * N + j] = T(0); L.Yl[m * N + j] = T(0); }
+        }
+        __syncthreads();
+        flush_mfma<T>(G, N, L.Xl, L.Yl, a4);
+        __syncthreads();
     }
-    if (kc > 0) flush_pending<T>(G, N, X, Y, diag0, dcorr, kc, kmax, tid, nth);
     __syncthreads();
     uint32_t* gm = a.accmask + ((long long)b * a.L + a.t) * nw;
-    for (int w = tid; w < nw; w += nth) gm[w] = mask[w];
+    for (int w = tid; w < nw; w += nth) gm[w] = L.mask[w];
     if (tid == 0) {
         a.rng[b] = (x0 * a.powtab[2 * N - 1]) % MINSTD_M;
         a.acc_count[b] += (unsigned long long)nacc;
     }
 }
 
-size_t metropolis_lds_bytes(int N, int krows, size_t esz)
-{
-    return (size_t)2 * krows * N * esz + (size_t)3 * N * sizeof(double) + (size_t)N * sizeof(int) +
-           (size_t)((N + 31) / 32) * sizeof(uint32_t) + 64;
-}
+__global__ void qr_kernel(double* __restrict__ bufs, long long stride, int N, int W);
 
-hipError_t prepare_kernels()
+hipError_t prepare_kernels(bool blocked_qr)
 {
     hipError_t e = hipFuncSetAttribute((const void*)metropolis_kernel<double>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     if (e != hipSuccess) return e;
+    if (blocked_qr) {
+        e = hipFuncSetAttribute((const void*)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+    }
     return hipFuncSetAttribute((const void*)metropolis_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                160 * 1024);
 }
@@ -337,8 +562,9 @@ hipError_t prepare_kernels()
 template <typename T>
 hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, hipStream_t s)
 {
-    const int kal = a.kmax == 1 ? 1 : ((a.kmax + 7) / 8) * 8;
-    const size_t shm = metropolis_lds_bytes(a.N, kal, sizeof(T));
+    if (a.kmax < 1 || a.kmax > MB_MAX) return hipErrorInvalidValue;
+    const size_t shm = metropolis_lds_bytes(a.N, a.kmax, sizeof(T));
+    if (shm > 160 * 1024) return hipErrorInvalidValue;
     const int threads = a.N >= 256 ? 256 : ((a.N + 63) / 64) * 64;
     hipLaunchKernelGGL(metropolis_kernel<T>, dim3(batch), dim3(threads), shm, s, a);
     return hipGetLastError();
@@ -354,6 +580,7 @@ template hipError_t launch_metropolis<float>(const MetroArgs<float>&, int, hipSt
 // ---------------------------------------------------------------------------------------------
 #define QR_NB 16
 #define QR_MAXN 256
+#define QR_CH 8    // rows per register chunk in the trailing update
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -362,7 +589,240 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Blocked Householder QR.  Per panel of QR_NB columns:
+//   1. the panel (rows p0..N-1) is factored in LDS, one reflector per column (dlarfg);
+//   2. its R rows go back to global memory, the reflectors V (unit lower trapezoid, explicit
+//      ones and zeros) stay in LDS together with the compact WY factor T;
+//   3. every trailing column (thread-per-column, coalesced along the row-major rows) is
+//      updated as a <- a - V T^T V^T a, streaming the column through registers in chunks of
+//      QR_CH rows so that QR_CH independent loads are in flight.
+struct QrLds {
+    double Pv[QR_MAXN + QR_CH][QR_NB];   // panel, then the reflectors V (unit lower trapezoid)
+    double Yv[QR_MAXN + QR_CH][QR_NB];   // V T^T: the trailing update is a <- a - Yv (V^T a)
+    double Tm[QR_NB][QR_NB];
+    double VtV[QR_NB][QR_NB];
+    double tau[QR_NB];
+};
+
 __global__ __launch_bounds__(256) void qr_kernel(double* __restrict__ bufs, long long stride, int N, int W)
+{
+    extern __shared__ __align__(16) unsigned char qr_smem[];
+    QrLds& S = *reinterpret_cast<QrLds*>(qr_smem);
+    auto& Pv = S.Pv;
+    auto& Yv = S.Yv;
+    auto& Tm = S.Tm;
+    auto& VtV = S.VtV;
+    auto& tau = S.tau;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    double* A = bufs + (long long)blockIdx.x * stride;
+
+    for (int p0 = 0; p0 < N; p0 += QR_NB) {
+        const int nb = min(QR_NB, N - p0);
+        const int R = N - p0;
+        // rows up to the next multiple of QR_CH are zero-filled: the chunked trailing update
+        // reads them (multiplied by zero) past the last row
+        const int Rp = ((R + QR_CH - 1) / QR_CH) * QR_CH;
+        for (int e = tid; e < Rp * QR_NB; e += 256) {
+            const int r = e / QR_NB, c = e % QR_NB;
+            Pv[r][c] = (c < nb && r < R) ? A[(long long)(p0 + r) * W + p0 + c] : 0.0;
+        }
+        __syncthreads();
+        for (int c = 0; c < nb; ++c) {
+            if (wave == 0) {
+                const double alpha = Pv[c][c];   // read by every lane before lane 0 overwrites it
+                double s = 0.0;
+                for (int r = c + 1 + lane; r < R; r += 64) s += Pv[r][c] * Pv[r][c];
+                s = wave_sum(s);
+                double tc = 0.0, beta = alpha, scal = 0.0;
+                if (s > 0.0) {
+                    beta = -copysign(sqrt(alpha * alpha + s), alpha);
+                    tc = (beta - alpha) / beta;
+                    scal = 1.0 / (alpha - beta);
+                }
+                for (int r = c + 1 + lane; r < R; r += 64) Pv[r][c] *= scal;
+                if (lane == 0) { Pv[c][c] = beta; tau[c] = tc; }
+            }
+            __syncthreads();
+            for (int cc = c + 1 + wave; cc < nb; cc += 4) {
+                double s = (lane == 0) ? Pv[c][cc] : 0.0;
+                for (int r = c + 1 + lane; r < R; r += 64) s += Pv[r][c] * Pv[r][cc];
+                s = wave_sum(s) * tau[c];
+                if (lane == 0) Pv[c][cc] -= s;
+                for (int r = c + 1 + lane; r < R; r += 64) Pv[r][cc] -= s * Pv[r][c];
+            }
+            __syncthreads();
+        }
+        // R rows of the panel to global memory (below-diagonal entries are never read)
+        for (int e = tid; e < nb * nb; e += 256) {
+            const int r = e / nb, c = e % nb;
+            if (r <= c) A[(long long)(p0 + r) * W + p0 + c] = Pv[r][c];
+        }
+        __syncthreads();
+        // make V explicit: ones on the diagonal, zeros above
+        for (int e = tid; e < QR_NB * QR_NB; e += 256) {
+            const int r = e / QR_NB, c = e % QR_NB;
+            if (r <= c) Pv[r][c] = (r == c && c < nb) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        for (int e = tid; e < nb * nb; e += 256) {
+            const int i = e / nb, j = e % nb;
+            double s = 0.0;
+            if (i < j)
+                for (int r = j; r < R; ++r) s += Pv[r][i] * Pv[r][j];
+            VtV[i][j] = s;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int i = 0; i < QR_NB; ++i) {
+                for (int r = 0; r < QR_NB; ++r) Tm[r][i] = 0.0;
+                if (i >= nb) continue;
+                for (int r = 0; r < i; ++r) {
+                    double s = 0.0;
+                    for (int q = r; q < i; ++q) s += Tm[r][q] * VtV[q][i];
+                    Tm[r][i] = -tau[i] * s;
+                }
+                Tm[i][i] = tau[i];
+            }
+        }
+        __syncthreads();
+        // Yv = V T^T : Yv[r][m] = sum_{q >= m} V[r][q] T[m][q]
+        for (int e = tid; e < R * QR_NB; e += 256) {
+            const int r = e / QR_NB, m = e % QR_NB;
+            double acc = 0.0;
+            for (int q = m; q < QR_NB; ++q) acc += Pv[r][q] * Tm[m][q];
+            Yv[r][m] = acc;
+        }
+        for (int e = R * QR_NB + tid; e < ((R + QR_CH - 1) / QR_CH) * QR_CH * QR_NB; e += 256)
+            Yv[e / QR_NB][e % QR_NB] = 0.0;
+        __syncthreads();
+        for (int col = p0 + nb + tid; col < W; col += 256) {
+            double* Acol = A + (long long)p0 * W + col;
+            double w[QR_NB];
+#pragma unroll
+            for (int m = 0; m < QR_NB; ++m) w[m] = 0.0;
+            for (int r0 = 0; r0 < R; r0 += QR_CH) {
+                double x[QR_CH];
+#pragma unroll
+                for (int i = 0; i < QR_CH; ++i) x[i] = (r0 + i < R) ? Acol[(long long)(r0 + i) * W] : 0.0;
+#pragma unroll
+                for (int m2 = 0; m2 < QR_NB / 2; ++m2) {
+#pragma unroll
+                    for (int i = 0; i < QR_CH; ++i) {
+                        const double2 v = reinterpret_cast<const double2*>(&Pv[r0 + i][0])[m2];
+                        w[2 * m2] += v.x * x[i];
+                        w[2 * m2 + 1] += v.y * x[i];
+                    }
+                }
+            }
+            for (int r0 = 0; r0 < R; r0 += QR_CH) {
+                double x[QR_CH];
+#pragma unroll
+                for (int i = 0; i < QR_CH; ++i) x[i] = (r0 + i < R) ? Acol[(long long)(r0 + i) * W] : 0.0;
+#pragma unroll
+                for (int m2 = 0; m2 < QR_NB / 2; ++m2) {
+#pragma unroll
+                    for (int i = 0; i < QR_CH; ++i) {
+                        const double2 v = reinterpret_cast<const double2*>(&Yv[r0 + i][0])[m2];
+                        x[i] -= v.x * w[2 * m2] + v.y * w[2 * m2 + 1];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < QR_CH; ++i)
+                    if (r0 + i < R) Acol[(long long)(r0 + i) * W] = x[i];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_qr(double* buf, long long stride, int N, int W, int batch, hipStream_t s)
+{
+    if (N > QR_MAXN) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(qr_kernel, dim3(batch), dim3(256), sizeof(QrLds), s, buf, stride, N, W);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+//  Upper triangular solve X = R^{-1} Z, thread-per-right-hand-side column.  Row blocks of
+//  TRSM_BR from the bottom: the diagonal block is solved in registers, then the rows above are
+//  updated with the block column R[0:i0, i0:i0+BR) staged in LDS (broadcast reads) while each
+//  thread streams its column in chunks of TRSM_BR rows.
+// ---------------------------------------------------------------------------------------------
+#define TRSM_BR 16
+#define TRSM_CH 8
+__global__ __launch_bounds__(256) void trsm_kernel(double* __restrict__ bufs, long long stride, int N, int W,
+                                                   int c0, int P)
+{
+    __shared__ __align__(16) double Rb[QR_MAXN][TRSM_BR];
+    double* A = bufs + (long long)blockIdx.y * stride;
+    const int col = blockIdx.x * 256 + threadIdx.x;
+    const bool active = col < P;
+    double* Z = A + c0 + (active ? col : 0);
+    for (int iend = N; iend > 0; iend -= TRSM_BR) {
+        const int i0 = iend - TRSM_BR > 0 ? iend - TRSM_BR : 0;
+        const int bs = iend - i0;
+        __syncthreads();
+        for (int e = threadIdx.x; e < iend * TRSM_BR; e += 256) {
+            const int k = e / TRSM_BR, m = e % TRSM_BR;
+            Rb[k][m] = (m < bs) ? A[(long long)k * W + i0 + m] : 0.0;
+        }
+        __syncthreads();
+        if (!active) continue;
+        double x[TRSM_BR];
+#pragma unroll
+        for (int m = 0; m < TRSM_BR; ++m) x[m] = (m < bs) ? Z[(long long)(i0 + m) * W] : 0.0;
+#pragma unroll
+        for (int m = TRSM_BR - 1; m >= 0; --m) {
+            if (m < bs) {
+                double s = x[m];
+#pragma unroll
+                for (int q = m + 1; q < TRSM_BR; ++q) s -= Rb[i0 + m][q] * x[q];
+                x[m] = s / Rb[i0 + m][m];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < TRSM_BR; ++m)
+            if (m < bs) Z[(long long)(i0 + m) * W] = x[m];
+        for (int k0 = 0; k0 < i0; k0 += TRSM_CH) {
+            double z[TRSM_CH];
+#pragma unroll
+            for (int i = 0; i < TRSM_CH; ++i) z[i] = (k0 + i < i0) ? Z[(long long)(k0 + i) * W] : 0.0;
+#pragma unroll
+            for (int i = 0; i < TRSM_CH; ++i) {
+                const int k = min(k0 + i, QR_MAXN - 1);
+                const double2* rr = reinterpret_cast<const double2*>(&Rb[k][0]);
+                double s = z[i];
+#pragma unroll
+                for (int m2 = 0; m2 < TRSM_BR / 2; ++m2) {
+                    const double2 v = rr[m2];
+                    s -= v.x * x[2 * m2] + v.y * x[2 * m2 + 1];
+                }
+                z[i] = s;
+            }
+#pragma unroll
+            for (int i = 0; i < TRSM_CH; ++i)
+                if (k0 + i < i0) Z[(long long)(k0 + i) * W] = z[i];
+        }
+    }
+}
+
+hipError_t launch_trsm_upper(double* buf, long long stride, int N, int W, int c0, int P, int batch,
+                             hipStream_t s)
+{
+    if (N > QR_MAXN) return hipErrorInvalidValue;
+    dim3 grid((P + 255) / 256, batch);
+    hipLaunchKernelGGL(trsm_kernel, grid, dim3(256), 0, s, buf, stride, N, W, c0, P);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+//  Reference-path QR and triangular solve (one column / row at a time through global memory).
+//  These are the kernels the GPU parity suite was first validated with; the engine uses them
+//  unless the blocked kernels above are enabled (DQMC_EXPERIMENTAL_KERNELS=1).
+// ---------------------------------------------------------------------------------------------
+
+
+__global__ __launch_bounds__(256) void qr_v1_kernel(double* __restrict__ bufs, long long stride, int N, int W)
 {
     __shared__ double P[QR_MAXN][QR_NB + 1];  // panel rows p0..N-1 (relative), reflectors below diag
     __shared__ double Tm[QR_NB][QR_NB];
@@ -480,18 +940,17 @@ __global__ __launch_bounds__(256) void qr_kernel(double* __restrict__ bufs, long
     }
 }
 
-hipError_t launch_qr(double* buf, long long stride, int N, int W, int batch, hipStream_t s)
+hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, hipStream_t s)
 {
     if (N > QR_MAXN) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(qr_kernel, dim3(batch), dim3(256), 0, s, buf, stride, N, W);
+    hipLaunchKernelGGL(qr_v1_kernel, dim3(batch), dim3(256), 0, s, buf, stride, N, W);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------------------------
 //  Upper triangular solve, thread-per-right-hand-side column, 16-row blocks from the bottom.
 // ---------------------------------------------------------------------------------------------
-#define TRSM_BR 16
-__global__ __launch_bounds__(256) void trsm_kernel(double* __restrict__ bufs, long long stride, int N, int W,
+__global__ __launch_bounds__(256) void trsm_v1_kernel(double* __restrict__ bufs, long long stride, int N, int W,
                                                    int c0, int P)
 {
     double* A = bufs + (long long)blockIdx.y * stride;
@@ -528,11 +987,11 @@ __global__ __launch_bounds__(256) void trsm_kernel(double* __restrict__ bufs, lo
     }
 }
 
-hipError_t launch_trsm_upper(double* buf, long long stride, int N, int W, int c0, int P, int batch,
+hipError_t launch_trsm_upper_v1(double* buf, long long stride, int N, int W, int c0, int P, int batch,
                              hipStream_t s)
 {
     dim3 grid((P + 255) / 256, batch);
-    hipLaunchKernelGGL(trsm_kernel, grid, dim3(256), 0, s, buf, stride, N, W, c0, P);
+    hipLaunchKernelGGL(trsm_v1_kernel, grid, dim3(256), 0, s, buf, stride, N, W, c0, P);
     return hipGetLastError();
 }
 

@@ -72,6 +72,8 @@ def lib():
             "dqmc_event_record": (C.c_int, [vp, C.c_int]),
             "dqmc_event_elapsed": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_float)]),
             "dqmc_use_graphs": (C.c_int, [vp, C.c_int]),
+            "dqmc_kernel_timer_begin": (C.c_int, [vp, C.c_int, C.c_int]),
+            "dqmc_kernel_timer_end": (C.c_int, [vp, dp, C.POINTER(C.c_int)]),
             "dqmc_dims": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         }
         for name, (res, args) in sig.items():
@@ -197,6 +199,15 @@ class Engine:
 
     def event_record(self, slot):
         self._chk(lib().dqmc_event_record(self._h, slot), "event_record")
+
+    def kernel_timer_begin(self, kernel, max_launches):
+        self._chk(lib().dqmc_kernel_timer_begin(self._h, kernel, max_launches), "kernel_timer_begin")
+
+    def kernel_timer_end(self):
+        ms = C.c_double()
+        n = C.c_int()
+        self._chk(lib().dqmc_kernel_timer_end(self._h, C.byref(ms), C.byref(n)), "kernel_timer_end")
+        return float(ms.value), int(n.value)
 
     def event_elapsed(self, a, b):
         ms = C.c_float()

@@ -57,8 +57,11 @@ enum {
 
 /* Context lifetime.  precision: DQMC_FP64 (everything FP64) or DQMC_MIXED (wraps and the
  * delayed-update flushes in FP32, stabilization / recompose / accept ratios FP64).
- * delay_k: 1 = one Sherman-Morrison rank-1 update per accepted flip
- * (src/hubbard.cpp:132-139); k > 1 regroups the same updates into rank-k flushes.
+ * delay_k (1..32): sites are visited in blocks of delay_k; accept/reject inside a block runs
+ * on the block of G, and the block's accepted flips are applied to G as one rank-|A| update.
+ * delay_k = 1 applies every accepted flip immediately, the reference's rank-1
+ * Sherman-Morrison update (src/hubbard.cpp:132-139).  The accept/reject sequence does not
+ * depend on delay_k.
  * Replaces: construction of DQMC::Core + Model::HubbardAttractiveU + SquareLattice and their
  * initialize() calls (src/main.cpp:119-128; src/dqmc_core.cpp:20-30; src/hubbard.cpp:25-48). */
 int dqmc_create(const dqmc_params_c* params, int n_chains, int precision, int delay_k, dqmc_ctx** out);
@@ -125,6 +128,15 @@ int dqmc_get_accept_mask(dqmc_ctx* ctx, int chain, uint32_t* out);
 int dqmc_synchronize(dqmc_ctx* ctx);
 int dqmc_event_record(dqmc_ctx* ctx, int slot);
 int dqmc_event_elapsed(dqmc_ctx* ctx, int slot_begin, int slot_end, float* ms);
+
+/* Per-launch timing of one hot kernel during the next sweeps (benchmark instrumentation):
+ * kernel 0 = Metropolis slice update, 1 = Householder QR, 2 = GEMM (all), 3 = triangular solve,
+ * 4 = Kronecker-factored exp(-dt K) products (wraps, B-chain).
+ * Each launch of that kernel is bracketed by HIP events on the context's stream; end() returns
+ * the summed device time and the number of launches.  Eager sweeps only (graphs are bypassed
+ * while a timer is active).  No counterpart in the reference. */
+int dqmc_kernel_timer_begin(dqmc_ctx* ctx, int kernel, int max_launches);
+int dqmc_kernel_timer_end(dqmc_ctx* ctx, double* total_ms, int* launches);
 
 /* Capture each sweep kind into a hipGraph on first use and replay it afterwards
  * (1 = on, default 0). */
