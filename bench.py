@@ -3,8 +3,10 @@
 
 Workload (one "step" = one directional updating sweep of every chain of the batch):
   16x16 square lattice (N=256), nnt=1, u=4 (U=-4), mu=0, beta=8, nt=80 (dt=0.1),
-  stabilization every 10 slices, fields Bernoulli(0.5) from seeds base+c, FP64 G,
-  delayed rank-k updates, B independent chains batched on one GPU.
+  stabilization every 8 slices (the largest pace whose FP64 recompute error stays below
+  1e-6 over all chains at this beta*|K|; at pace 10 the FP64 wrap error of the worst chain
+  reaches ~3e-6), fields Bernoulli(0.5) from seeds base+c, FP64 G, delayed rank-k updates,
+  B independent chains batched on one GPU.
 Metric: aggregate sweeps/s = B * steps / wall seconds of the timed region (inputs resident).
 Also measured in the same run:
   * the plain FP64 GPU form (1 chain, rank-1 Sherman-Morrison updates),
@@ -23,7 +25,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hubbard-attractive-u_amd"))
 
-NL, NT, BETA, U, MU, PACE = 16, 80, 8.0, 4.0, 0.0, 10
+NL, NT, BETA, U, MU, PACE = 16, 80, 8.0, 4.0, 0.0, 8
 BASE_SEED = 20240520
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
@@ -126,7 +128,9 @@ def main():
     log("[bench] value %.2f sweeps/s (%.3f s for %d steps)" % (value, dt, args.steps))
 
     # roofline: live per-launch timing of the dominant kernel over extra timed sweeps
-    kernel_names = {0: "metropolis_kernel<double>", 1: "qr_kernel", 2: "gemm_kernel<double>", 3: "trsm_kernel",
+    blocked = os.environ.get("DQMC_EXPERIMENTAL_KERNELS", "") == "1"
+    kernel_names = {0: "metropolis_kernel<double>", 1: "qr_kernel" if blocked else "qr_v1_kernel",
+                    2: "gemm_kernel<double>", 3: "trsm_kernel" if blocked else "trsm_v1_kernel",
                     4: "kron_left/right_kernel<double>"}
     prof = {}
     for kid in (0, 1, 2, 3, 4):
@@ -190,7 +194,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (random Ising fields from fixed seeds)",
-        "config": {"workload": "dqmc-sweep 16x16 U=-4 mu=0 beta=8 L=80 pace=10", "chains": args.chains,
+        "config": {"workload": "dqmc-sweep 16x16 U=-4 mu=0 beta=8 L=80 pace=8", "chains": args.chains,
                    "delay_k": args.delay, "precision": "fp64", "graphs": not args.no_graphs},
         "max_abs_dG": dG,
         "dG_ok": dG <= 1e-6,

@@ -774,6 +774,23 @@ static int check_chain(dqmc_ctx* c, int chain)
     return 0;
 }
 
+int dqmc_set_collect(dqmc_ctx* c, int eq, int dyn)
+{
+    if (!c) return DQMC_EINVAL;
+    if (c->initialized) return c->fail(DQMC_ESTATE, "set_collect after init");
+    const long long nn = c->NN();
+    const size_t B = c->B, L = c->L;
+    c->eqtime = eq != 0;
+    c->dynamic = dyn != 0;
+    if ((c->eqtime || c->dynamic) && !c->vecgftt) RCHECK(c->alloc(&c->vecgftt, B * L * nn * 8));
+    if (c->dynamic && !c->vecgft0) {
+        RCHECK(c->alloc(&c->vecgft0, B * L * nn * 8)); RCHECK(c->alloc(&c->vecgf0t, B * L * nn * 8));
+        RCHECK(c->alloc(&c->gt0A, B * nn * 8)); RCHECK(c->alloc(&c->gt0B, B * nn * 8));
+        RCHECK(c->alloc(&c->g0tA, B * nn * 8)); RCHECK(c->alloc(&c->g0tB, B * nn * 8));
+    }
+    return 0;
+}
+
 int dqmc_set_chain_mu(dqmc_ctx* c, int chain, double mu)
 {
     RCHECK(check_chain(c, chain));

@@ -53,6 +53,7 @@ def lib():
             "dqmc_destroy": (None, [vp]),
             "dqmc_last_error": (C.c_char_p, [vp]),
             "dqmc_set_chain_mu": (C.c_int, [vp, C.c_int, C.c_double]),
+            "dqmc_set_collect": (C.c_int, [vp, C.c_int, C.c_int]),
             "dqmc_set_seed": (C.c_int, [vp, C.c_int, C.c_uint64]),
             "dqmc_get_rng_state": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint64)]),
             "dqmc_fields_random": (C.c_int, [vp, C.c_int]),
@@ -125,6 +126,9 @@ class Engine:
     # --- configuration -----------------------------------------------------------------------
     def set_chain_mu(self, chain, mu):
         self._chk(lib().dqmc_set_chain_mu(self._h, chain, mu), "set_chain_mu")
+
+    def set_collect(self, eqtime, dynamic):
+        self._chk(lib().dqmc_set_collect(self._h, int(eqtime), int(dynamic)), "set_collect")
 
     def set_seed(self, chain, seed):
         self._chk(lib().dqmc_set_seed(self._h, chain, int(seed)), "set_seed")

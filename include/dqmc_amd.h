@@ -68,6 +68,11 @@ int dqmc_create(const dqmc_params_c* params, int n_chains, int precision, int de
 void dqmc_destroy(dqmc_ctx* ctx);
 const char* dqmc_last_error(dqmc_ctx* ctx);
 
+/* Switch on the storage of G(t,t) / G(t,0), G(0,t) for measurements after creation (the
+ * reference decides this in Core::initialize from the measurement handle,
+ * src/dqmc_core.cpp:28-29 and :43-65).  Allocates on first use; call before dqmc_init. */
+int dqmc_set_collect(dqmc_ctx* ctx, int collect_eqtime, int collect_dynamic);
+
 /* Per-chain chemical potential (the batched workload spreads mu over chains).
  * Must be called before dqmc_init. */
 int dqmc_set_chain_mu(dqmc_ctx* ctx, int chain, double mu);
