@@ -69,6 +69,7 @@ struct dqmc_ctx {
     std::vector<uint64_t> rng_host;
     std::string err;
     int tile64 = TILE_64x64, tile32 = TILE_64x64;
+    float tune_ms[2][TILE_COUNT] = {};      // autotuner timings per precision and tile
 
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
@@ -97,6 +98,11 @@ struct dqmc_ctx {
     double *vecgftt = nullptr, *vecgft0 = nullptr, *vecgf0t = nullptr;
     double *wrap_eq = nullptr, *wrap_dyn = nullptr, *maxdG = nullptr;
     DStack left, right;
+    // parameter adjoint at fixed fields (allocated on first use)
+    int* nb = nullptr;
+    int8_t* fields_save = nullptr;
+    double *adj_partial = nullptr, *adj_grad = nullptr, *adj_gdiag = nullptr, *adj_glogdet = nullptr,
+           *adj_out = nullptr;
 
     long long NN() const { return (long long)N * N; }
     long long sqr() const { return (long long)N * 3 * N; }
@@ -573,6 +579,85 @@ int do_sweep_dynamic(dqmc_ctx* c)
     return 0;
 }
 
+// Core::initialize_svdstacks + initialize_green_functions (src/dqmc_core.cpp:67-97) from the
+// current fields: right stack = B-chain blocks from beta down to 0, G = G(0).  On return the
+// QR buffer still holds the factored (1 + Ur Dr Vr^T) system of the full chain.
+int build_from_fields(dqmc_ctx* c)
+{
+    const int L = c->L;
+    const long long nn = c->NN();
+    c->left.len = 0; c->right.len = 0; c->current_t = 0;
+    RCHECK(set_identity_stk(c));
+    for (int t = L; t >= 1; --t) {
+        RCHECK(chain_mul(c, t % L, true));
+        if ((t - 1) % c->pace == 0) {
+            RCHECK(stack_push(c, c->right));
+            RCHECK(set_identity_stk(c));
+        }
+    }
+    RCHECK(recompute_eq(c, c->W3));
+    HCHECK(c, launch_copy_d(c->W3, nn, c->G64, nn, nn, c->B, c->stream));
+    if (c->prec == DQMC_MIXED) HCHECK(c, launch_copy_d2f(c->W3, nn, c->G32, nn, nn, c->B, c->stream));
+    return 0;
+}
+
+// Buffers of the fixed-field forward and the adjoint: the dynamic G vectors and the
+// neighbour table of the hopping matrix (square_lattice.cpp:30-61: directions x+1, y+1, x-1, y-1).
+int ensure_adjoint_buffers(dqmc_ctx* c)
+{
+    const long long nn = c->NN();
+    const size_t B = c->B, L = c->L, N = c->N;
+    if (!c->vecgftt) RCHECK(c->alloc(&c->vecgftt, B * L * nn * 8));
+    if (!c->vecgft0) {
+        RCHECK(c->alloc(&c->vecgft0, B * L * nn * 8)); RCHECK(c->alloc(&c->vecgf0t, B * L * nn * 8));
+        RCHECK(c->alloc(&c->gt0A, B * nn * 8)); RCHECK(c->alloc(&c->gt0B, B * nn * 8));
+        RCHECK(c->alloc(&c->g0tA, B * nn * 8)); RCHECK(c->alloc(&c->g0tB, B * nn * 8));
+    }
+    if (!c->nb) {
+        RCHECK(c->alloc(&c->nb, N * 4 * sizeof(int)));
+        RCHECK(c->alloc(&c->fields_save, B * L * N));
+        RCHECK(c->alloc(&c->adj_partial, B * (L + 1) * 3 * 8));
+        RCHECK(c->alloc(&c->adj_grad, B * 3 * 8));
+        RCHECK(c->alloc(&c->adj_gdiag, B * L * N * 8));
+        RCHECK(c->alloc(&c->adj_glogdet, B * 8));
+        RCHECK(c->alloc(&c->adj_out, B * 8));
+        std::vector<int> nb(N * 4);
+        const int nl = c->nl;
+        for (int i = 0; i < (int)N; ++i) {
+            const int x = i % nl, y = i / nl;
+            nb[4 * i + 0] = (x + 1) % nl + nl * y;
+            nb[4 * i + 1] = x + nl * ((y + 1) % nl);
+            nb[4 * i + 2] = (x - 1 + nl) % nl + nl * y;
+            nb[4 * i + 3] = x + nl * ((y - 1 + nl) % nl);
+        }
+        HCHECK(c, hipMemcpy(c->nb, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+// The run statistics are a property of the Markov chain; the auxiliary passes of the forward
+// and the adjoint leave them as they were.
+struct SavedStats {
+    std::vector<double> we, wd, dg;
+};
+int save_stats(dqmc_ctx* c, SavedStats& s)
+{
+    s.we.resize(c->B); s.wd.resize(c->B); s.dg.resize(c->B);
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(s.we.data(), c->wrap_eq, c->B * 8, hipMemcpyDeviceToHost));
+    HCHECK(c, hipMemcpy(s.wd.data(), c->wrap_dyn, c->B * 8, hipMemcpyDeviceToHost));
+    HCHECK(c, hipMemcpy(s.dg.data(), c->maxdG, c->B * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+int restore_stats(dqmc_ctx* c, const SavedStats& s)
+{
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(c->wrap_eq, s.we.data(), c->B * 8, hipMemcpyHostToDevice));
+    HCHECK(c, hipMemcpy(c->wrap_dyn, s.wd.data(), c->B * 8, hipMemcpyHostToDevice));
+    HCHECK(c, hipMemcpy(c->maxdG, s.dg.data(), c->B * 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
 // Run a sweep eagerly, or through a captured graph keyed by its kind and store flag.
 typedef int (*sweep_fn)(dqmc_ctx*);
 int run_sweep(dqmc_ctx* c, sweep_fn fn, int kind)
@@ -633,6 +718,7 @@ int autotune(dqmc_ctx* c)
                 HCHECK(c, hipEventSynchronize(e1));
                 float ms = 0;
                 HCHECK(c, hipEventElapsedTime(&ms, e0, e1));
+                if (rep == 1) c->tune_ms[p][tile] = ms / 3;
                 if (rep == 1 && ms < best) { best = ms; bt = tile; }
             }
         }
@@ -870,22 +956,9 @@ int dqmc_init(dqmc_ctx* c)
     HCHECK(c, launch_fill_d(c->wrap_dyn, B, 0.0, c->stream));
     HCHECK(c, launch_fill_d(c->maxdG, B, 0.0, c->stream));
     if (!c->initialized) RCHECK(autotune(c));
-    // Core::initialize_svdstacks (src/dqmc_core.cpp:67-87)
-    c->left.len = 0; c->right.len = 0; c->current_t = 0;
-    RCHECK(set_identity_stk(c));
-    for (int t = L; t >= 1; --t) {
-        RCHECK(chain_mul(c, t % L, true));
-        if ((t - 1) % c->pace == 0) {
-            RCHECK(stack_push(c, c->right));
-            RCHECK(set_identity_stk(c));
-        }
-    }
-    // Core::initialize_green_functions (:89-97)
-    RCHECK(recompute_eq(c, c->W3));
-    HCHECK(c, launch_copy_d(c->W3, nn, c->G64, nn, nn, B, c->stream));
-    if (c->prec == DQMC_MIXED) HCHECK(c, launch_copy_d2f(c->W3, nn, c->G32, nn, nn, B, c->stream));
+    RCHECK(build_from_fields(c));
     HCHECK(c, hipStreamSynchronize(c->stream));
-    (void)N;
+    (void)N; (void)L; (void)nn;
     c->initialized = true;
     return 0;
 }
@@ -1070,6 +1143,90 @@ int dqmc_use_graphs(dqmc_ctx* c, int on)
     if (!c) return DQMC_EINVAL;
     c->use_graphs = on != 0;
     return 0;
+}
+
+int dqmc_get_tuning(dqmc_ctx* c, int* tile_fp64, int* tile_fp32, float* ms)
+{
+    if (!c) return DQMC_EINVAL;
+    if (!c->initialized) return c->fail(DQMC_ESTATE, "dqmc_get_tuning: the autotuner runs in dqmc_init");
+    if (tile_fp64) *tile_fp64 = c->tile64;
+    if (tile_fp32) *tile_fp32 = c->tile32;
+    if (ms)
+        for (int p = 0; p < 2; ++p)
+            for (int t = 0; t < TILE_COUNT; ++t) ms[p * TILE_COUNT + t] = c->tune_ms[p][t];
+    return 0;
+}
+
+int dqmc_set_tuning(dqmc_ctx* c, int tile_fp64, int tile_fp32)
+{
+    if (!c) return DQMC_EINVAL;
+    if (tile_fp64 < 0 || tile_fp64 >= TILE_COUNT || tile_fp32 < 0 || tile_fp32 >= TILE_COUNT)
+        return c->fail(DQMC_EINVAL, "dqmc_set_tuning: tile index out of range");
+    if (!c->initialized) return c->fail(DQMC_ESTATE, "dqmc_set_tuning: call after dqmc_init");
+    if (!c->graphs.empty()) return c->fail(DQMC_ESTATE, "dqmc_set_tuning: sweeps were already captured");
+    c->tile64 = tile_fp64;
+    c->tile32 = tile_fp32;
+    return 0;
+}
+
+int dqmc_forward_fixed(dqmc_ctx* c, double* logdet, double* gdiag)
+{
+    if (!c) return DQMC_EINVAL;
+    if (!c->initialized) return c->fail(DQMC_ESTATE, "dqmc_forward_fixed before dqmc_init");
+    if (c->prec != DQMC_FP64) return c->fail(DQMC_EINVAL, "dqmc_forward_fixed needs an FP64 context");
+    const int B = c->B, N = c->N, L = c->L;
+    const long long nn = c->NN();
+    RCHECK(ensure_adjoint_buffers(c));
+    SavedStats st;
+    RCHECK(save_stats(c, st));
+    RCHECK(build_from_fields(c));
+    HCHECK(c, launch_logdet_00bb(c->qr, c->sqr(), 2 * N, sD(c, c->right, c->right.len - 1), (long long)c->S * N, N,
+                                 c->adj_out, B, c->stream));
+    RCHECK(do_sweep_dynamic(c));
+    HCHECK(c, launch_gf_diag(c->vecgftt, (long long)L * nn, L, N, c->adj_gdiag, B, c->stream));
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    if (logdet) HCHECK(c, hipMemcpy(logdet, c->adj_out, (size_t)B * 8, hipMemcpyDeviceToHost));
+    if (gdiag) HCHECK(c, hipMemcpy(gdiag, c->adj_gdiag, (size_t)B * L * N * 8, hipMemcpyDeviceToHost));
+    RCHECK(build_from_fields(c));
+    return restore_stats(c, st);
+}
+
+int dqmc_adjoint(dqmc_ctx* c, const double* gbar_logdet, const double* gbar_gdiag, double* grad)
+{
+    if (!c || !grad) return DQMC_EINVAL;
+    if (!c->initialized) return c->fail(DQMC_ESTATE, "dqmc_adjoint before dqmc_init");
+    if (c->prec != DQMC_FP64) return c->fail(DQMC_EINVAL, "dqmc_adjoint needs an FP64 context");
+    const int B = c->B, N = c->N, L = c->L;
+    const long long nn = c->NN();
+    RCHECK(ensure_adjoint_buffers(c));
+    SavedStats st;
+    RCHECK(save_stats(c, st));
+    std::vector<double> zl;
+    if (!gbar_logdet) { zl.assign(B, 0.0); gbar_logdet = zl.data(); }
+    HCHECK(c, hipMemcpy(c->adj_glogdet, gbar_logdet, (size_t)B * 8, hipMemcpyHostToDevice));
+    if (gbar_gdiag) HCHECK(c, hipMemcpy(c->adj_gdiag, gbar_gdiag, (size_t)B * L * N * 8, hipMemcpyHostToDevice));
+    else HCHECK(c, launch_fill_d(c->adj_gdiag, (long long)B * L * N, 0.0, c->stream));
+    HCHECK(c, launch_fill_d(c->adj_grad, (long long)B * 3, 0.0, c->stream));
+    HCHECK(c, hipMemcpyAsync(c->fields_save, c->fields, (size_t)B * L * N, hipMemcpyDeviceToDevice, c->stream));
+    // d alpha / d u for cosh(alpha) = e^{dt u / 2} (src/hubbard.cpp:35)
+    const double dalpha = c->dt * std::exp(0.5 * c->dt * c->u) / (2.0 * std::sinh(c->alpha));
+    for (int tau = 0; tau < L; ++tau) {
+        HCHECK(c, launch_roll_fields(c->fields_save, c->fields, L, N, tau, B, c->stream));
+        RCHECK(build_from_fields(c));
+        RCHECK(do_sweep_dynamic(c));
+        AdjointArgs a;
+        a.gf0t = c->vecgf0t; a.gft0 = c->vecgft0; a.gftt = c->vecgftt; a.sv = (long long)L * nn;
+        a.gbar_diag = c->adj_gdiag; a.gbar_logdet = c->adj_glogdet; a.fields = c->fields; a.nb = c->nb;
+        a.partial = c->adj_partial; a.grad = c->adj_grad;
+        a.N = N; a.L = L; a.tau = tau;
+        a.alpha = c->alpha; a.dt = c->dt; a.dalpha = dalpha;
+        HCHECK(c, launch_adjoint_contract(a, B, c->stream));
+    }
+    HCHECK(c, hipMemcpyAsync(c->fields, c->fields_save, (size_t)B * L * N, hipMemcpyDeviceToDevice, c->stream));
+    RCHECK(build_from_fields(c));
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(grad, c->adj_grad, (size_t)B * 3 * 8, hipMemcpyDeviceToHost));
+    return restore_stats(c, st);
 }
 
 int dqmc_dims(dqmc_ctx* c, int* ns, int* nt, int* nch)

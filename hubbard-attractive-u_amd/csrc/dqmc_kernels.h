@@ -149,4 +149,29 @@ hipError_t launch_axpi(const double* src, long long ss, double* dst, long long s
                        int batch, hipStream_t st);
 hipError_t launch_fill_d(double* dst, long long n, double v, hipStream_t st);
 
+// ---- parameter adjoint at fixed fields (dqmc_adjoint.hip) ----------------------------------
+struct AdjointArgs {
+    const double* gf0t;         // [batch][L] N x N: G(tau, tau+t') of the rolled pass
+    const double* gft0;         // [batch][L] N x N: G(tau+t', tau)
+    const double* gftt;         // [batch][L] N x N: slice 0 holds G(tau)
+    long long sv;               // chain stride of the three (L*N*N)
+    const double* gbar_diag;    // [batch][L][N] cotangent of G(t)_ii (original slice order)
+    const double* gbar_logdet;  // [batch] cotangent of log|det(1 + B_L..B_1)|, may be null
+    const int8_t* fields;       // rolled fields [batch][L][N]
+    const int* nb;              // [N][4] neighbours (x+1, y+1, x-1, y-1)
+    double* partial;            // [batch][L+1][3] scratch
+    double* grad;               // [batch][3] accumulated d/d(nnt, mu, u)
+    int N, L, tau;
+    double alpha, dt, dalpha;   // dalpha = d alpha / d u
+};
+// Contracts one rolled pass into grad (two launches: per-slice partials, fixed-order sum).
+hipError_t launch_adjoint_contract(const AdjointArgs& a, int batch, hipStream_t s);
+// dst[b][k][i] = src[b][(k + shift) % L][i]
+hipError_t launch_roll_fields(const int8_t* src, int8_t* dst, int L, int N, int shift, int batch, hipStream_t s);
+// log|det(1 + u diag(d) v^T)| per chain from the factored launch_build_00bb system.
+hipError_t launch_logdet_00bb(const double* buf, long long sbuf, int W, const double* d, long long sd, int N,
+                              double* out, int batch, hipStream_t s);
+// out[b][t][i] = vec_b[t](i, i)
+hipError_t launch_gf_diag(const double* vec, long long sv, int L, int N, double* out, int batch, hipStream_t s);
+
 }  // namespace dqmc

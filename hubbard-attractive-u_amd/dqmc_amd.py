@@ -75,6 +75,10 @@ def lib():
             "dqmc_use_graphs": (C.c_int, [vp, C.c_int]),
             "dqmc_kernel_timer_begin": (C.c_int, [vp, C.c_int, C.c_int]),
             "dqmc_kernel_timer_end": (C.c_int, [vp, dp, C.POINTER(C.c_int)]),
+            "dqmc_get_tuning": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+            "dqmc_set_tuning": (C.c_int, [vp, C.c_int, C.c_int]),
+            "dqmc_forward_fixed": (C.c_int, [vp, dp, dp]),
+            "dqmc_adjoint": (C.c_int, [vp, dp, dp, dp]),
             "dqmc_dims": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
         }
         for name, (res, args) in sig.items():
@@ -212,6 +216,41 @@ class Engine:
         n = C.c_int()
         self._chk(lib().dqmc_kernel_timer_end(self._h, C.byref(ms), C.byref(n)), "kernel_timer_end")
         return float(ms.value), int(n.value)
+
+    def tuning(self):
+        """Autotuned GEMM tiles {'fp64': name, 'fp32': name, 'ms': {(prec, name): ms}}."""
+        a, b = C.c_int(), C.c_int()
+        ms = (C.c_float * 6)()
+        self._chk(lib().dqmc_get_tuning(self._h, C.byref(a), C.byref(b), ms), "get_tuning")
+        names = ["64x64", "32x32", "64x32"]
+        return {"fp64": names[a.value], "fp32": names[b.value],
+                "ms": {(p, names[t]): ms[i * 3 + t] for i, p in enumerate(("fp64", "fp32")) for t in range(3)}}
+
+    def forward_fixed(self):
+        """(logdet [chains], gdiag [chains, nt, ns]) at the current fields, no updates."""
+        ld = np.zeros(self.n_chains)
+        gd = np.zeros((self.n_chains, self.nt, self.n))
+        dp = C.POINTER(C.c_double)
+        self._chk(lib().dqmc_forward_fixed(self._h, ld.ctypes.data_as(dp), gd.ctypes.data_as(dp)), "forward_fixed")
+        return ld, gd
+
+    def adjoint(self, gbar_logdet=None, gbar_gdiag=None):
+        """[chains, 3] = cotangents contracted with d/d(nnt, mu, u) of forward_fixed's outputs."""
+        dp = C.POINTER(C.c_double)
+        gl = gg = None
+        if gbar_logdet is not None:
+            gl = np.ascontiguousarray(gbar_logdet, dtype=np.float64).reshape(self.n_chains)
+        if gbar_gdiag is not None:
+            gg = np.ascontiguousarray(gbar_gdiag, dtype=np.float64).reshape(self.n_chains, self.nt, self.n)
+        out = np.zeros((self.n_chains, 3))
+        self._chk(lib().dqmc_adjoint(self._h, gl.ctypes.data_as(dp) if gl is not None else None,
+                                     gg.ctypes.data_as(dp) if gg is not None else None, out.ctypes.data_as(dp)),
+                  "adjoint")
+        return out
+
+    def set_tuning(self, fp64, fp32):
+        names = ["64x64", "32x32", "64x32"]
+        self._chk(lib().dqmc_set_tuning(self._h, names.index(fp64), names.index(fp32)), "set_tuning")
 
     def event_elapsed(self, a, b):
         ms = C.c_float()

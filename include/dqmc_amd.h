@@ -147,6 +147,34 @@ int dqmc_kernel_timer_end(dqmc_ctx* ctx, double* total_ms, int* launches);
  * (1 = on, default 0). */
 int dqmc_use_graphs(dqmc_ctx* ctx, int on);
 
+/* GEMM tile chosen by the autotuner in dqmc_init for FP64 (and FP32) products:
+ * 0 = 64x64, 1 = 32x32, 2 = 64x32 output tiles; ms (optional, 2 x 3 floats) receives the
+ * measured time per wrap-shaped product of every candidate.  No counterpart in the reference. */
+int dqmc_get_tuning(dqmc_ctx* ctx, int* tile_fp64, int* tile_fp32, float* ms);
+/* Override the autotuner's choice (after dqmc_init, before the first captured sweep). */
+int dqmc_set_tuning(dqmc_ctx* ctx, int tile_fp64, int tile_fp32);
+
+/* Fixed-field forward of the differentiable quantities (FP64 contexts).  For every chain, at
+ * its current fields and without Metropolis updates:
+ *   logdet[b]          = log|det(1 + B_L ... B_1)|  (the fermion determinant of one spin
+ *                        species; the reference's weight is its square times the field
+ *                        factor e^{-alpha sum s}, src/hubbard.cpp:123-130),
+ *   gdiag[b][t][i]     = G(t)(i, i) for the equal-time G of slice t in the storage order of
+ *                        Core::m_vecgftt (src/dqmc_core.cpp:370-382).
+ * Either output may be NULL.  Leaves the context as dqmc_init does (G recomputed, run
+ * statistics unchanged).  No counterpart in the reference. */
+int dqmc_forward_fixed(dqmc_ctx* ctx, double* logdet, double* gdiag);
+
+/* Adjoint of dqmc_forward_fixed with respect to (nnt, mu, u) (FP64 contexts):
+ *   grad[b][0..2] = gbar_logdet[b] * d logdet[b]/d(nnt, mu, u)
+ *                   + sum_{t,i} gbar_gdiag[b][t][i] * d G_b(t)(i,i)/d(nnt, mu, u)
+ * at fixed fields (alpha follows u through cosh(alpha) = e^{dt u/2}).  Either cotangent may be
+ * NULL (zero).  The derivative of G(t) is contracted through the time-displaced G of all slice
+ * pairs, one stabilized dynamic pass per slice (cost ~ L dynamic sweeps).  The d/du component
+ * diverges as u -> 0 (alpha ~ sqrt(dt u)).  Leaves the context as dqmc_init does.
+ * No counterpart in the reference. */
+int dqmc_adjoint(dqmc_ctx* ctx, const double* gbar_logdet, const double* gbar_gdiag, double* grad);
+
 /* Lattice size / chain count accessors. */
 int dqmc_dims(dqmc_ctx* ctx, int* ns, int* nt, int* n_chains);
 

@@ -64,3 +64,25 @@ def greens_np(expk, alpha, fields, t=0):
     for l in list(range(t + 1, L + 1)) + list(range(1, t + 1)):
         M = bmat(expk, alpha, fields[l % L]) @ M
     return np.linalg.inv(np.eye(n) + M)
+
+
+def logdet_mp(expk, alpha, fields, dps=120):
+    """log|det(1 + B(L)...B(1))| at `dps` digits on the FP64 inputs; fields is [L][N]."""
+    L, n = fields.shape
+    with mp.workdps(dps):
+        E = _to_mp(expk)
+        M = mp.eye(n)
+        for l in list(range(1, L + 1)):
+            d = [mp.e ** (mp.mpf(alpha) * int(fields[l % L][i])) for i in range(n)]
+            EM = E * M
+            for i in range(n):
+                for j in range(n):
+                    EM[i, j] *= d[i]
+            M = EM
+        return float(mp.log(abs(mp.det(mp.eye(n) + M))))
+
+
+def square_lattice_bands(nl):
+    """a_k = 2 (cos kx + cos ky): eigenvalues of the adjacency (periodic images counted)."""
+    ks = [2 * math.pi * m / nl for m in range(nl)]
+    return np.array([2.0 * (math.cos(kx) + math.cos(ky)) for kx in ks for ky in ks])
