@@ -129,7 +129,9 @@ def main():
 
     # roofline: live per-launch timing of the dominant kernel over extra timed sweeps
     blocked = os.environ.get("DQMC_EXPERIMENTAL_KERNELS", "") == "1"
-    kernel_names = {0: "metropolis_kernel<double>", 1: "qr_kernel" if blocked else "qr_v1_kernel",
+    qr_name = "qr_kernel" if blocked else ("qr_v1_kernel" if os.environ.get("DQMC_QR", "") == "v1"
+                                           else "qr_blocked (qr_panel_kernel + 2 gemm_kernel<double> per 32 columns)")
+    kernel_names = {0: "metropolis_kernel<double>", 1: qr_name,
                     2: "gemm_kernel<double>", 3: "trsm_kernel" if blocked else "trsm_v1_kernel",
                     4: "kron_left/right_kernel<double>"}
     prof = {}

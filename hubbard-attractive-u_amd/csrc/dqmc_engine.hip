@@ -63,6 +63,10 @@ struct dqmc_ctx {
     bool initialized = false;
     // blocked QR / LDS triangular solve / Kronecker wraps: opt-in (DQMC_EXPERIMENTAL_KERNELS=1)
     bool experimental = false;
+    // QR of the stabilizations: 0 = blocked (panel kernel + MFMA trailing GEMMs), 1 = one-kernel
+    // column-panel QR (DQMC_QR=v1)
+    int qr_mode = 0;
+    double *qrV = nullptr, *qrZ = nullptr, *qrWt = nullptr;
     int current_t = 0;
     long long updating_sweeps = 0;
     std::vector<double> chain_mu;
@@ -239,7 +243,9 @@ int gemm(dqmc_ctx* c, const T* A, long long sA, int lda, int tA, const T* B, lon
 
 hipError_t qr_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int batch, hipStream_t s)
 {
-    return c->experimental ? launch_qr(buf, stride, N, W, batch, s) : launch_qr_v1(buf, stride, N, W, batch, s);
+    if (c->experimental) return launch_qr(buf, stride, N, W, batch, s);
+    if (c->qr_mode == 1) return launch_qr_v1(buf, stride, N, W, batch, s);
+    return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrZ, c->qrWt, 3LL * N * 32, c->tile64, s);
 }
 hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int c0, int P, int batch, hipStream_t s)
 {
@@ -759,6 +765,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     {
         const char* ex = std::getenv("DQMC_EXPERIMENTAL_KERNELS");
         c->experimental = ex && ex[0] == '1';
+        const char* q = std::getenv("DQMC_QR");
+        c->qr_mode = (q && std::string(q) == "v1") ? 1 : 0;
     }
     c->chain_mu.assign(n_chains, p->mu);
     c->rng_host.assign(n_chains, 1ULL);
@@ -788,6 +796,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     RCHECK(c->alloc(&c->W3, (size_t)B * nn * 8));
     RCHECK(c->alloc(&c->qr, (size_t)B * c->sqr() * 8));
     RCHECK(c->alloc(&c->perm, (size_t)B * N * 4));
+    RCHECK(c->alloc(&c->qrV, (size_t)B * N * 32 * 8)); RCHECK(c->alloc(&c->qrZ, (size_t)B * N * 32 * 8));
+    RCHECK(c->alloc(&c->qrWt, (size_t)B * 3 * N * 32 * 8));
     for (DStack* s : {&c->left, &c->right}) {
         RCHECK(c->alloc(&s->U, (size_t)B * S * nn * 8));
         RCHECK(c->alloc(&s->V, (size_t)B * S * nn * 8));

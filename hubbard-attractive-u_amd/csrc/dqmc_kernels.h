@@ -93,6 +93,12 @@ hipError_t launch_qr(double* buf, long long stride, int N, int W, int batch, hip
 
 // Same contracts, reference-path implementations (see the kernel file).
 hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, hipStream_t s);
+// Blocked Householder QR of the same row-major N x W system (N <= 256): per panel of 32 columns
+// one factorization launch (one workgroup per chain) and two MFMA GEMM launches for the trailing
+// columns.  Scratch per chain: Vs and Zs N*32 doubles each, Wt wt_stride >= W*32 doubles.
+// `tile` is the GEMM tile of the trailing update.
+hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Zs,
+                             double* Wt, long long wt_stride, int tile, hipStream_t s);
 hipError_t launch_trsm_upper_v1(double* buf, long long stride, int N, int W, int c0, int P, int batch,
                                 hipStream_t s);
 

@@ -948,6 +948,213 @@ hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, 
 }
 
 // ---------------------------------------------------------------------------------------------
+//  Blocked Householder QR in three launches per panel of QRB columns:
+//    qr_panel_kernel  one workgroup per chain keeps the panel rows in registers (panel row r in
+//                     thread r), factors it column by column and builds the compact-WY factor T
+//                     from the same block reductions; writes R back, V and Z^T = V T^T out.
+//    GEMM 1 (MFMA)    Wt = At^T V         (At = trailing columns of the panel rows)
+//    GEMM 2 (MFMA)    At^T -= Wt Z,  i.e. At <- (I - V T^T V^T) At = Q_panel^T At
+//  The trailing columns include the right-hand sides, so they receive Q^T rhs.
+// ---------------------------------------------------------------------------------------------
+#define QRB 32
+
+// Sum over the block of K values per thread by recursive halving: after the 5 exchange steps
+// each lane pair owns one index, so one wave needs K shuffles instead of 6 K.  Every thread
+// returns the full sums in v[0..K-1].
+template <int K>
+__device__ __forceinline__ void block_sum_vec(double (&v)[K], double (*red)[K])
+{
+    static_assert(K == 32, "recursive halving is written for 32 values");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double w[16];
+    {
+        const bool hi = lane & 32;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const double keep = hi ? v[m + 16] : v[m], send = hi ? v[m] : v[m + 16];
+            w[m] = keep + __shfl_xor(send, 32, 64);
+        }
+    }
+    {
+        const bool hi = lane & 16;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const double keep = hi ? w[m + 8] : w[m], send = hi ? w[m] : w[m + 8];
+            w[m] = keep + __shfl_xor(send, 16, 64);
+        }
+    }
+    {
+        const bool hi = lane & 8;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const double keep = hi ? w[m + 4] : w[m], send = hi ? w[m] : w[m + 4];
+            w[m] = keep + __shfl_xor(send, 8, 64);
+        }
+    }
+    {
+        const bool hi = lane & 4;
+#pragma unroll
+        for (int m = 0; m < 2; ++m) {
+            const double keep = hi ? w[m + 2] : w[m], send = hi ? w[m] : w[m + 2];
+            w[m] = keep + __shfl_xor(send, 4, 64);
+        }
+    }
+    {
+        const bool hi = lane & 2;
+        const double keep = hi ? w[1] : w[0], send = hi ? w[0] : w[1];
+        w[0] = keep + __shfl_xor(send, 2, 64);
+    }
+    w[0] += __shfl_xor(w[0], 1, 64);
+    const int idx = ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 + ((lane >> 2) & 1) * 2 +
+                    ((lane >> 1) & 1);
+    if ((lane & 1) == 0) red[wave][idx] = w[0];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < K; ++m) v[m] = red[0][m] + red[1][m] + red[2][m] + red[3][m];
+    __syncthreads();
+}
+
+__device__ __forceinline__ double block_sum_1(double v, double* red)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    const double r = red[0] + red[1] + red[2] + red[3];
+    __syncthreads();
+    return r;
+}
+
+// Factor columns [p0, p0+nb) of the row-major N x W block (rows p0..N-1, at most 256 of them),
+// nb <= QRB.  V (column-major, ld N, rows relative to p0) gets the unit lower trapezoidal
+// reflectors, Zt (column-major, ld N) gets Z^T = V T^T with H_1..H_nb = I - V T V^T.
+__global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs, long long stride, int N, int W,
+                                                       int p0, int nb, double* __restrict__ Vs,
+                                                       double* __restrict__ Zs)
+{
+    __shared__ double red[4][QRB];
+    __shared__ double red1[4];
+    __shared__ double Tm[QRB][QRB + 1];
+    __shared__ double alpha_s;
+    const int r = threadIdx.x;
+    const int b = blockIdx.x;
+    const int R = N - p0;
+    const bool live = r < R;
+    double* A = bufs + (long long)b * stride;
+    double* V = Vs + (long long)b * N * QRB;
+    double* Zt = Zs + (long long)b * N * QRB;
+
+    for (int e = r; e < QRB * (QRB + 1); e += 256) (&Tm[0][0])[e] = 0.0;
+    double x[QRB];
+#pragma unroll
+    for (int m = 0; m < QRB; ++m) x[m] = (live && m < nb) ? A[(long long)(p0 + r) * W + p0 + m] : 0.0;
+    __syncthreads();
+
+    for (int i = 0; i < nb; ++i) {
+        double xi = 0.0;
+#pragma unroll
+        for (int m = 0; m < QRB; ++m)
+            if (m == i) xi = x[m];
+        if (r == i) alpha_s = xi;
+        const double s2 = block_sum_1((live && r > i) ? xi * xi : 0.0, red1);
+        const double alpha = alpha_s;
+        double beta = alpha, tau = 0.0, scal = 0.0;
+        if (s2 > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        const double vr = (r == i) ? 1.0 : ((live && r > i) ? xi * scal : 0.0);
+        const double xnew = (r == i) ? beta : ((r > i) ? vr : xi);
+#pragma unroll
+        for (int m = 0; m < QRB; ++m)
+            if (m == i) x[m] = xnew;
+        // dots: y_m = v_m . v_i (m < i) for T, d_m = v_i . x_m (m > i) for the panel update
+        double d[QRB];
+#pragma unroll
+        for (int m = 0; m < QRB; ++m) {
+            double vm = 0.0;
+            if (m < i) vm = (r == m) ? 1.0 : (r > m ? x[m] : 0.0);
+            else if (m > i && m < nb) vm = x[m];
+            d[m] = vm * vr;
+        }
+        block_sum_vec<QRB>(d, red);
+#pragma unroll
+        for (int m = 0; m < QRB; ++m)
+            if (m > i && m < nb) x[m] -= tau * vr * d[m];
+        // T(0:i, i) = -tau T(0:i, 0:i) y ; T(i, i) = tau
+        if (r < i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < QRB; ++k)
+                if (k >= r && k < i) acc += Tm[r][k] * d[k];
+            Tm[r][i] = -tau * acc;
+        } else if (r == i) {
+            Tm[i][i] = tau;
+        }
+    }
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < QRB; ++m) {
+            if (m < nb) {
+                A[(long long)(p0 + r) * W + p0 + m] = (r <= m) ? x[m] : 0.0;
+                V[r + (long long)m * N] = (r == m) ? 1.0 : (r > m ? x[m] : 0.0);
+            }
+        }
+        // Z^T(r, m) = sum_{k >= m} T(m, k) v_k(r)
+#pragma unroll
+        for (int m = 0; m < QRB; ++m) {
+            if (m < nb) {
+                double z = 0.0;
+#pragma unroll
+                for (int k = 0; k < QRB; ++k) {
+                    if (k >= m && k < nb) {
+                        const double vk = (r == k) ? 1.0 : (r > k ? x[k] : 0.0);
+                        z += Tm[m][k] * vk;
+                    }
+                }
+                Zt[r + (long long)m * N] = z;
+            }
+        }
+    }
+}
+
+// scratch: V and Zt [batch][N * QRB], Wt [batch][wt_stride] with wt_stride >= (W - N) * QRB... (W * QRB)
+hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Zs,
+                             double* Wt, long long wt_stride, int tile, hipStream_t s)
+{
+    if (N > 256 || N < 1 || W <= N) return hipErrorInvalidValue;
+    if (wt_stride < (long long)W * QRB) return hipErrorInvalidValue;
+    for (int p0 = 0; p0 < N; p0 += QRB) {
+        const int nb = (N - p0 < QRB) ? N - p0 : QRB;
+        const int R = N - p0, cols = W - (p0 + nb);
+        hipLaunchKernelGGL(qr_panel_kernel, dim3(batch), dim3(256), 0, s, buf, stride, N, W, p0, nb, Vs, Zs);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if (cols <= 0) continue;
+        double* At = buf + (p0 + nb) + (long long)p0 * W;   // At^T: cols x R, column-major, ld W
+        GemmArgs<double> g1;
+        g1.M = cols; g1.Nc = nb; g1.K = R;
+        g1.A = At; g1.sA = stride; g1.lda = W; g1.tA = 0;
+        g1.B = Vs; g1.sB = (long long)N * QRB; g1.ldb = N; g1.tB = 0;
+        g1.C = Wt; g1.sC = wt_stride; g1.ldc = W; g1.tC = 0;
+        g1.alpha = 1.0; g1.beta = 0.0; g1.batch = batch;
+        e = launch_gemm<double>(g1, TILE_64x32, s);
+        if (e != hipSuccess) return e;
+        GemmArgs<double> g2;
+        g2.M = cols; g2.Nc = R; g2.K = nb;
+        g2.A = Wt; g2.sA = wt_stride; g2.lda = W; g2.tA = 0;
+        g2.B = Zs; g2.sB = (long long)N * QRB; g2.ldb = N; g2.tB = 1;
+        g2.C = At; g2.sC = stride; g2.ldc = W; g2.tC = 0;
+        g2.alpha = -1.0; g2.beta = 1.0; g2.batch = batch;
+        e = launch_gemm<double>(g2, tile, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------------------------
 //  Upper triangular solve, thread-per-right-hand-side column, 16-row blocks from the bottom.
 // ---------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void trsm_v1_kernel(double* __restrict__ bufs, long long stride, int N, int W,
