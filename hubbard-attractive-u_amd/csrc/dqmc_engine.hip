@@ -66,7 +66,11 @@ struct dqmc_ctx {
     // QR of the stabilizations: 0 = blocked (panel kernel + MFMA trailing GEMMs), 1 = one-kernel
     // column-panel QR (DQMC_QR=v1)
     int qr_mode = 0;
-    double *qrV = nullptr, *qrZ = nullptr, *qrWt = nullptr;
+    // slice products exp(-+dt K) X as Kronecker squares of the ring exponential (8 MFMAs per
+    // lattice vector) instead of dense N x N GEMMs (DQMC_WRAP=dense selects the GEMMs)
+    bool kron = true;
+    bool trsm_v1 = false;   // DQMC_TRSM=v1: one-kernel thread-per-column back-substitution
+    double *qrV = nullptr, *qrT = nullptr, *qrWt = nullptr;   // blocked-QR scratch: V, T, Wt and Wt T
     int current_t = 0;
     long long updating_sweeps = 0;
     std::vector<double> chain_mu;
@@ -245,10 +249,11 @@ hipError_t qr_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int 
 {
     if (c->experimental) return launch_qr(buf, stride, N, W, batch, s);
     if (c->qr_mode == 1) return launch_qr_v1(buf, stride, N, W, batch, s);
-    return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrZ, c->qrWt, 3LL * N * 32, c->tile64, s);
+    return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrT, c->qrWt, 3LL * N * 32, c->tile64, s);
 }
 hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int c0, int P, int batch, hipStream_t s)
 {
+    if (!c->experimental && !c->trsm_v1) return launch_trsm_blocked(buf, stride, N, W, c0, P, batch, c->tile64, s);
     return c->experimental ? launch_trsm_upper(buf, stride, N, W, c0, P, batch, s)
                            : launch_trsm_upper_v1(buf, stride, N, W, c0, P, batch, s);
 }
@@ -274,7 +279,7 @@ int wrap_up(dqmc_ctx* c, int t)
 {
     const long long nn = c->NN();
     const int N = c->N;
-    if (c->experimental) {
+    if (c->kron) {
         // D (E(x)E) G (F(x)F) D^{-1}: the chemical potential sits in the D tables
         if (c->prec == DQMC_MIXED) {
             RCHECK(kron<float>(c, true, c->G32, c->T32, c->E1f, dfield(c, c->tab, t)));
@@ -301,7 +306,7 @@ int wrap_down(dqmc_ctx* c, int t)
 {
     const long long nn = c->NN();
     const int N = c->N;
-    if (c->experimental) {
+    if (c->kron) {
         // (F(x)F) D^{-1} G D (E(x)E)
         if (c->prec == DQMC_MIXED) {
             RCHECK(kron<float>(c, false, c->G32, c->T32, c->E1f, dnone(), dfield(c, c->tab, t)));
@@ -326,7 +331,7 @@ int left_expk(dqmc_ctx* c, const double* X, double* C, const double* table, int 
 {
     const long long nn = c->NN();
     const DiagArg d = dfield(c, table, t);
-    if (c->experimental) return kron<double>(c, true, X, C, table == c->tab ? c->E1 : c->F1, inner ? dnone() : d, inner ? d : dnone());
+    if (c->kron) return kron<double>(c, true, X, C, table == c->tab ? c->E1 : c->F1, inner ? dnone() : d, inner ? d : dnone());
     return gemm<double>(c, table == c->tab ? c->expK : c->invExpK, 0, c->N, 0, X, nn, c->N, 0, C, nn, c->N, 0,
                         inner ? dnone() : d, inner ? d : dnone());
 }
@@ -336,7 +341,7 @@ int right_expk(dqmc_ctx* c, const double* X, double* C, const double* table, int
     const long long nn = c->NN();
     const DiagArg d = dfield(c, table, t);
     const bool fwd = table == c->tab;   // tab goes with exp(-dt K), itab with exp(+dt K)
-    if (c->experimental)
+    if (c->kron)
         return kron<double>(c, false, X, C, fwd ? c->E1 : c->F1, dnone(), inner ? d : dnone(), inner ? dnone() : d);
     return gemm<double>(c, X, nn, c->N, 0, fwd ? c->expK : c->invExpK, 0, c->N, 0, C, nn, c->N, 0, dnone(),
                         inner ? d : dnone(), inner ? dnone() : d);
@@ -767,6 +772,10 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         c->experimental = ex && ex[0] == '1';
         const char* q = std::getenv("DQMC_QR");
         c->qr_mode = (q && std::string(q) == "v1") ? 1 : 0;
+        const char* tr = std::getenv("DQMC_TRSM");
+        c->trsm_v1 = tr && std::string(tr) == "v1";
+        const char* w = std::getenv("DQMC_WRAP");
+        c->kron = !(w && std::string(w) == "dense");
     }
     c->chain_mu.assign(n_chains, p->mu);
     c->rng_host.assign(n_chains, 1ULL);
@@ -796,8 +805,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     RCHECK(c->alloc(&c->W3, (size_t)B * nn * 8));
     RCHECK(c->alloc(&c->qr, (size_t)B * c->sqr() * 8));
     RCHECK(c->alloc(&c->perm, (size_t)B * N * 4));
-    RCHECK(c->alloc(&c->qrV, (size_t)B * N * 32 * 8)); RCHECK(c->alloc(&c->qrZ, (size_t)B * N * 32 * 8));
-    RCHECK(c->alloc(&c->qrWt, (size_t)B * 3 * N * 32 * 8));
+    RCHECK(c->alloc(&c->qrV, (size_t)B * N * 32 * 8)); RCHECK(c->alloc(&c->qrT, (size_t)B * 32 * 32 * 8));
+    RCHECK(c->alloc(&c->qrWt, (size_t)2 * B * 3 * N * 32 * 8));
     for (DStack* s : {&c->left, &c->right}) {
         RCHECK(c->alloc(&s->U, (size_t)B * S * nn * 8));
         RCHECK(c->alloc(&s->V, (size_t)B * S * nn * 8));

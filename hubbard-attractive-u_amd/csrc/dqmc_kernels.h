@@ -94,11 +94,16 @@ hipError_t launch_qr(double* buf, long long stride, int N, int W, int batch, hip
 // Same contracts, reference-path implementations (see the kernel file).
 hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, hipStream_t s);
 // Blocked Householder QR of the same row-major N x W system (N <= 256): per panel of 32 columns
-// one factorization launch (one workgroup per chain) and two MFMA GEMM launches for the trailing
-// columns.  Scratch per chain: Vs and Zs N*32 doubles each, Wt wt_stride >= W*32 doubles.
-// `tile` is the GEMM tile of the trailing update.
-hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Zs,
+// one factorization launch (one workgroup per chain) and three MFMA GEMM launches for the
+// trailing columns.  Scratch per chain: Vs N*32 doubles, Ts 32*32 doubles, and Wt holding two
+// blocks of wt_stride >= W*32 doubles (batch * wt_stride apart).  `tile` is the GEMM tile of
+// the trailing update.
+hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
                              double* Wt, long long wt_stride, int tile, hipStream_t s);
+// Blocked in-place back-substitution of the same system: 32-row diagonal solves (thread per
+// right-hand side) and MFMA GEMM updates of the rows above (`tile` = GEMM tile).
+hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,
+                               hipStream_t s);
 hipError_t launch_trsm_upper_v1(double* buf, long long stride, int N, int W, int c0, int P, int batch,
                                 hipStream_t s);
 

@@ -395,33 +395,50 @@ size_t metropolis_lds_bytes(int N, int kb, size_t esz)
 
 // G(ri, cj) += sum_m X[ri][m] Y[m][cj] over the whole N x N matrix, on MFMA.  The tile is
 // computed transposed (MFMA rows = G columns) so that each lane group touches contiguous
-// column segments of the column-major G.
+// column segments of the column-major G.  Each wave keeps FT tiles in flight (all loads of a
+// group are issued before its first MFMA) so that the global-memory latency overlaps.
+#define FT 8
 template <typename T>
 __device__ void flush_mfma(T* __restrict__ G, int N, const T* __restrict__ Xl, const T* __restrict__ Yl, int a4)
 {
     typedef typename Mfma<T>::acc_t acc_t;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int nt = (N + 15) / 16;
-    for (int tile = wave; tile < nt * nt; tile += nw) {
-        const int c0 = (tile / nt) * 16, r0 = (tile % nt) * 16;   // G column block, G row block
-        const int gr = r0 + (lane & 15);
-        acc_t acc;
+    const int nt = (N + 15) / 16, ntt = nt * nt;
+    for (int t0 = wave * FT; t0 < ntt; t0 += nw * FT) {
+        acc_t acc[FT];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int gc = c0 + Mfma<T>::row(lane, r);
-            acc[r] = (gr < N && gc < N) ? G[gr + (long long)gc * N] : T(0);
+        for (int u = 0; u < FT; ++u) {
+            const int tile = t0 + u;
+            const int c0 = (tile / nt) * 16, r0 = (tile % nt) * 16;
+            const int gr = r0 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gc = c0 + Mfma<T>::row(lane, r);
+                acc[u][r] = (tile < ntt && gr < N && gc < N) ? G[gr + (long long)gc * N] : T(0);
+            }
         }
         for (int k0 = 0; k0 < a4; k0 += 4) {
             const int kk = k0 + (lane >> 4);
-            const int ca = c0 + (lane & 15);
-            const T av = (ca < N) ? Yl[kk * N + ca] : T(0);   // A(row = G column, k)
-            const T bv = (gr < N) ? Xl[kk * N + gr] : T(0);   // B(k, col = G row)
-            acc = Mfma<T>::mma(av, bv, acc);
+#pragma unroll
+            for (int u = 0; u < FT; ++u) {
+                const int tile = t0 + u;
+                const int c0 = (tile / nt) * 16, r0 = (tile % nt) * 16;
+                const int ca = c0 + (lane & 15), gr = r0 + (lane & 15);
+                const T av = (tile < ntt && ca < N) ? Yl[kk * N + ca] : T(0);   // A(row = G column, k)
+                const T bv = (tile < ntt && gr < N) ? Xl[kk * N + gr] : T(0);   // B(k, col = G row)
+                acc[u] = Mfma<T>::mma(av, bv, acc[u]);
+            }
         }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int gc = c0 + Mfma<T>::row(lane, r);
-            if (gr < N && gc < N) G[gr + (long long)gc * N] = acc[r];
+        for (int u = 0; u < FT; ++u) {
+            const int tile = t0 + u;
+            const int c0 = (tile / nt) * 16, r0 = (tile % nt) * 16;
+            const int gr = r0 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gc = c0 + Mfma<T>::row(lane, r);
+                if (tile < ntt && gr < N && gc < N) G[gr + (long long)gc * N] = acc[u][r];
+            }
         }
     }
 }
@@ -948,192 +965,175 @@ hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, 
 }
 
 // ---------------------------------------------------------------------------------------------
-//  Blocked Householder QR in three launches per panel of QRB columns:
-//    qr_panel_kernel  one workgroup per chain keeps the panel rows in registers (panel row r in
-//                     thread r), factors it column by column and builds the compact-WY factor T
-//                     from the same block reductions; writes R back, V and Z^T = V T^T out.
+//  Blocked Householder QR in four launches per panel of QRB columns:
+//    qr_panel4_kernel one 1024-thread workgroup per chain keeps the panel rows in registers,
+//                     factors it column by column (Householder, LAPACK dlarfg convention) and
+//                     forms the compact-WY factor T; writes R back, V and T out.
 //    GEMM 1 (MFMA)    Wt = At^T V         (At = trailing columns of the panel rows)
-//    GEMM 2 (MFMA)    At^T -= Wt Z,  i.e. At <- (I - V T^T V^T) At = Q_panel^T At
+//    GEMM 2 (MFMA)    Wt2 = Wt T
+//    GEMM 3 (MFMA)    At^T -= Wt2 V^T,  i.e. At <- (I - V T^T V^T) At = Q_panel^T At
 //  The trailing columns include the right-hand sides, so they receive Q^T rhs.
 // ---------------------------------------------------------------------------------------------
 #define QRB 32
 
-// Sum over the block of K values per thread by recursive halving: after the 5 exchange steps
-// each lane pair owns one index, so one wave needs K shuffles instead of 6 K.  Every thread
-// returns the full sums in v[0..K-1].
-template <int K>
-__device__ __forceinline__ void block_sum_vec(double (&v)[K], double (*red)[K])
+// Panel factorization with four threads per panel row (1024 threads; thread t handles row
+// t % 256 and the 8 panel columns 8 (t / 256) .. 8 (t / 256) + 7), so that each column step costs
+// each thread O(8) work and every SIMD holds four waves.  Writes R back, V (as above) and the
+// compact-WY factor T (column-major QRB x QRB per chain) with H_1..H_nb = I - V T V^T.
+#define QCPT 8
+__global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bufs, long long stride, int N, int W,
+                                                         int p0, int nb, double* __restrict__ Vs,
+                                                         double* __restrict__ Ts)
 {
-    static_assert(K == 32, "recursive halving is written for 32 values");
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double w[16];
-    {
-        const bool hi = lane & 32;
-#pragma unroll
-        for (int m = 0; m < 16; ++m) {
-            const double keep = hi ? v[m + 16] : v[m], send = hi ? v[m] : v[m + 16];
-            w[m] = keep + __shfl_xor(send, 32, 64);
-        }
-    }
-    {
-        const bool hi = lane & 16;
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const double keep = hi ? w[m + 8] : w[m], send = hi ? w[m] : w[m + 8];
-            w[m] = keep + __shfl_xor(send, 16, 64);
-        }
-    }
-    {
-        const bool hi = lane & 8;
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            const double keep = hi ? w[m + 4] : w[m], send = hi ? w[m] : w[m + 4];
-            w[m] = keep + __shfl_xor(send, 8, 64);
-        }
-    }
-    {
-        const bool hi = lane & 4;
-#pragma unroll
-        for (int m = 0; m < 2; ++m) {
-            const double keep = hi ? w[m + 2] : w[m], send = hi ? w[m] : w[m + 2];
-            w[m] = keep + __shfl_xor(send, 4, 64);
-        }
-    }
-    {
-        const bool hi = lane & 2;
-        const double keep = hi ? w[1] : w[0], send = hi ? w[0] : w[1];
-        w[0] = keep + __shfl_xor(send, 2, 64);
-    }
-    w[0] += __shfl_xor(w[0], 1, 64);
-    const int idx = ((lane >> 5) & 1) * 16 + ((lane >> 4) & 1) * 8 + ((lane >> 3) & 1) * 4 + ((lane >> 2) & 1) * 2 +
-                    ((lane >> 1) & 1);
-    if ((lane & 1) == 0) red[wave][idx] = w[0];
-    __syncthreads();
-#pragma unroll
-    for (int m = 0; m < K; ++m) v[m] = red[0][m] + red[1][m] + red[2][m] + red[3][m];
-    __syncthreads();
-}
-
-__device__ __forceinline__ double block_sum_1(double v, double* red)
-{
-    v = wave_sum(v);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    if (lane == 0) red[wave] = v;
-    __syncthreads();
-    const double r = red[0] + red[1] + red[2] + red[3];
-    __syncthreads();
-    return r;
-}
-
-// Factor columns [p0, p0+nb) of the row-major N x W block (rows p0..N-1, at most 256 of them),
-// nb <= QRB.  V (column-major, ld N, rows relative to p0) gets the unit lower trapezoidal
-// reflectors, Zt (column-major, ld N) gets Z^T = V T^T with H_1..H_nb = I - V T V^T.
-__global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs, long long stride, int N, int W,
-                                                       int p0, int nb, double* __restrict__ Vs,
-                                                       double* __restrict__ Zs)
-{
-    __shared__ double red[4][QRB];
-    __shared__ double red1[4];
-    __shared__ double Tm[QRB][QRB + 1];
-    __shared__ double alpha_s;
-    const int r = threadIdx.x;
+    __shared__ double red[16][QCPT];        // per-wave partial dot products
+    __shared__ double red1[16];             // per-wave partial norms
+    __shared__ double xcol[256];            // column i of the panel (before its reflection)
+    __shared__ double VtV[QRB][QRB + 1];    // VtV[k][i] = v_k . v_i, k < i
+    __shared__ double tau_s[QRB];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int g = t >> 8, r = t & 255;
     const int b = blockIdx.x;
     const int R = N - p0;
     const bool live = r < R;
     double* A = bufs + (long long)b * stride;
     double* V = Vs + (long long)b * N * QRB;
-    double* Zt = Zs + (long long)b * N * QRB;
+    double* Tg = Ts + (long long)b * QRB * QRB;
 
-    for (int e = r; e < QRB * (QRB + 1); e += 256) (&Tm[0][0])[e] = 0.0;
-    double x[QRB];
+    double x[QCPT];
 #pragma unroll
-    for (int m = 0; m < QRB; ++m) x[m] = (live && m < nb) ? A[(long long)(p0 + r) * W + p0 + m] : 0.0;
-    __syncthreads();
+    for (int m = 0; m < QCPT; ++m) {
+        const int col = g * QCPT + m;
+        x[m] = (live && col < nb) ? A[(long long)(p0 + r) * W + p0 + col] : 0.0;
+    }
 
     for (int i = 0; i < nb; ++i) {
+        const int go = i / QCPT, li = i % QCPT;
         double xi = 0.0;
 #pragma unroll
-        for (int m = 0; m < QRB; ++m)
-            if (m == i) xi = x[m];
-        if (r == i) alpha_s = xi;
-        const double s2 = block_sum_1((live && r > i) ? xi * xi : 0.0, red1);
-        const double alpha = alpha_s;
+        for (int m = 0; m < QCPT; ++m)
+            if (m == li) xi = x[m];
+        const bool owner = g == go;
+        if (owner) xcol[r] = live ? xi : 0.0;
+        double n2 = (owner && live && r > i) ? xi * xi : 0.0;
+        n2 = wave_sum(n2);
+        if (lane == 0) red1[wave] = n2;
+        __syncthreads();
+        double s2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) s2 += red1[go * 4 + w];
+        const double alpha = xcol[i];
         double beta = alpha, tau = 0.0, scal = 0.0;
         if (s2 > 0.0) {
             beta = -copysign(sqrt(alpha * alpha + s2), alpha);
             tau = (beta - alpha) / beta;
             scal = 1.0 / (alpha - beta);
         }
-        const double vr = (r == i) ? 1.0 : ((live && r > i) ? xi * scal : 0.0);
-        const double xnew = (r == i) ? beta : ((r > i) ? vr : xi);
+        const double vr = (r == i) ? 1.0 : ((live && r > i) ? xcol[r] * scal : 0.0);
+        if (owner) {
+            const double xnew = (r == i) ? beta : ((r > i) ? vr : xi);
 #pragma unroll
-        for (int m = 0; m < QRB; ++m)
-            if (m == i) x[m] = xnew;
-        // dots: y_m = v_m . v_i (m < i) for T, d_m = v_i . x_m (m > i) for the panel update
-        double d[QRB];
+            for (int m = 0; m < QCPT; ++m)
+                if (m == li) x[m] = xnew;
+        }
+        if (t == 0) tau_s[i] = tau;
+        // this thread's share of v_m . v_i (m < i) and v_i . x_m (m > i)
+        double d[QCPT];
 #pragma unroll
-        for (int m = 0; m < QRB; ++m) {
+        for (int m = 0; m < QCPT; ++m) {
+            const int col = g * QCPT + m;
             double vm = 0.0;
-            if (m < i) vm = (r == m) ? 1.0 : (r > m ? x[m] : 0.0);
-            else if (m > i && m < nb) vm = x[m];
+            if (col < i) vm = (r == col) ? 1.0 : (r > col ? x[m] : 0.0);
+            else if (col > i && col < nb) vm = x[m];
             d[m] = vm * vr;
         }
-        block_sum_vec<QRB>(d, red);
+        // recursive halving over the 64 lanes: 8 values -> lane owns index (lane >> 3) & 7
+        {
+            double w4[4], w2[2], w1;
+            const bool h32 = lane & 32, h16 = lane & 16, h8 = lane & 8;
 #pragma unroll
-        for (int m = 0; m < QRB; ++m)
-            if (m > i && m < nb) x[m] -= tau * vr * d[m];
-        // T(0:i, i) = -tau T(0:i, 0:i) y ; T(i, i) = tau
-        if (r < i) {
-            double acc = 0.0;
+            for (int m = 0; m < 4; ++m) {
+                const double keep = h32 ? d[m + 4] : d[m], send = h32 ? d[m] : d[m + 4];
+                w4[m] = keep + __shfl_xor(send, 32, 64);
+            }
 #pragma unroll
-            for (int k = 0; k < QRB; ++k)
-                if (k >= r && k < i) acc += Tm[r][k] * d[k];
-            Tm[r][i] = -tau * acc;
-        } else if (r == i) {
-            Tm[i][i] = tau;
+            for (int m = 0; m < 2; ++m) {
+                const double keep = h16 ? w4[m + 2] : w4[m], send = h16 ? w4[m] : w4[m + 2];
+                w2[m] = keep + __shfl_xor(send, 16, 64);
+            }
+            {
+                const double keep = h8 ? w2[1] : w2[0], send = h8 ? w2[0] : w2[1];
+                w1 = keep + __shfl_xor(send, 8, 64);
+            }
+            w1 += __shfl_xor(w1, 4, 64);
+            w1 += __shfl_xor(w1, 2, 64);
+            w1 += __shfl_xor(w1, 1, 64);
+            const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+            if ((lane & 7) == 0) red[wave][idx] = w1;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < QCPT; ++m) {
+            const int col = g * QCPT + m;
+            if (col > i && col < nb) {
+                const double dm = red[g * 4][m] + red[g * 4 + 1][m] + red[g * 4 + 2][m] + red[g * 4 + 3][m];
+                x[m] -= tau * vr * dm;
+            }
+        }
+        if (t < i) {
+            const int gk = t / QCPT, lk = t % QCPT;
+            VtV[t][i] = red[gk * 4][lk] + red[gk * 4 + 1][lk] + red[gk * 4 + 2][lk] + red[gk * 4 + 3][lk];
         }
     }
     __syncthreads();
     if (live) {
 #pragma unroll
-        for (int m = 0; m < QRB; ++m) {
-            if (m < nb) {
-                A[(long long)(p0 + r) * W + p0 + m] = (r <= m) ? x[m] : 0.0;
-                V[r + (long long)m * N] = (r == m) ? 1.0 : (r > m ? x[m] : 0.0);
-            }
-        }
-        // Z^T(r, m) = sum_{k >= m} T(m, k) v_k(r)
-#pragma unroll
-        for (int m = 0; m < QRB; ++m) {
-            if (m < nb) {
-                double z = 0.0;
-#pragma unroll
-                for (int k = 0; k < QRB; ++k) {
-                    if (k >= m && k < nb) {
-                        const double vk = (r == k) ? 1.0 : (r > k ? x[k] : 0.0);
-                        z += Tm[m][k] * vk;
-                    }
-                }
-                Zt[r + (long long)m * N] = z;
+        for (int m = 0; m < QCPT; ++m) {
+            const int col = g * QCPT + m;
+            if (col < nb) {
+                A[(long long)(p0 + r) * W + p0 + col] = (r <= col) ? x[m] : 0.0;
+                V[r + (long long)col * N] = (r == col) ? 1.0 : (r > col ? x[m] : 0.0);
             }
         }
     }
+    // T row j by thread j: T(j, j) = tau_j, T(j, i) = -tau_i sum_{k=j}^{i-1} T(j, k) VtV(k, i)
+    if (t < QRB) {
+        const int j = t;
+        double trow[QRB];
+#pragma unroll
+        for (int i = 0; i < QRB; ++i) {
+            double v = 0.0;
+            if (i == j && i < nb) {
+                v = tau_s[i];
+            } else if (i > j && i < nb) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < QRB; ++k)
+                    if (k >= j && k < i) acc += trow[k] * VtV[k][i];
+                v = -tau_s[i] * acc;
+            }
+            trow[i] = v;
+        }
+#pragma unroll
+        for (int i = 0; i < QRB; ++i) Tg[j + (long long)i * QRB] = trow[i];
+    }
 }
 
-// scratch: V and Zt [batch][N * QRB], Wt [batch][wt_stride] with wt_stride >= (W - N) * QRB... (W * QRB)
-hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Zs,
+hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
                              double* Wt, long long wt_stride, int tile, hipStream_t s)
 {
+    // per chain: Vs N*QRB, Ts QRB*QRB, Wt 2 * wt_stride (Wt and Wt T), wt_stride >= W*QRB
     if (N > 256 || N < 1 || W <= N) return hipErrorInvalidValue;
     if (wt_stride < (long long)W * QRB) return hipErrorInvalidValue;
+    double* Wt2 = Wt + (long long)batch * wt_stride;
     for (int p0 = 0; p0 < N; p0 += QRB) {
         const int nb = (N - p0 < QRB) ? N - p0 : QRB;
         const int R = N - p0, cols = W - (p0 + nb);
-        hipLaunchKernelGGL(qr_panel_kernel, dim3(batch), dim3(256), 0, s, buf, stride, N, W, p0, nb, Vs, Zs);
+        hipLaunchKernelGGL(qr_panel4_kernel, dim3(batch), dim3(1024), 0, s, buf, stride, N, W, p0, nb, Vs, Ts);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (cols <= 0) continue;
         double* At = buf + (p0 + nb) + (long long)p0 * W;   // At^T: cols x R, column-major, ld W
+        // Wt = At^T V
         GemmArgs<double> g1;
         g1.M = cols; g1.Nc = nb; g1.K = R;
         g1.A = At; g1.sA = stride; g1.lda = W; g1.tA = 0;
@@ -1142,13 +1142,86 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
         g1.alpha = 1.0; g1.beta = 0.0; g1.batch = batch;
         e = launch_gemm<double>(g1, TILE_64x32, s);
         if (e != hipSuccess) return e;
+        // Wt2 = Wt T
         GemmArgs<double> g2;
-        g2.M = cols; g2.Nc = R; g2.K = nb;
+        g2.M = cols; g2.Nc = nb; g2.K = nb;
         g2.A = Wt; g2.sA = wt_stride; g2.lda = W; g2.tA = 0;
-        g2.B = Zs; g2.sB = (long long)N * QRB; g2.ldb = N; g2.tB = 1;
-        g2.C = At; g2.sC = stride; g2.ldc = W; g2.tC = 0;
-        g2.alpha = -1.0; g2.beta = 1.0; g2.batch = batch;
-        e = launch_gemm<double>(g2, tile, s);
+        g2.B = Ts; g2.sB = (long long)QRB * QRB; g2.ldb = QRB; g2.tB = 0;
+        g2.C = Wt2; g2.sC = wt_stride; g2.ldc = W; g2.tC = 0;
+        g2.alpha = 1.0; g2.beta = 0.0; g2.batch = batch;
+        e = launch_gemm<double>(g2, TILE_64x32, s);
+        if (e != hipSuccess) return e;
+        // At^T -= Wt2 V^T
+        GemmArgs<double> g3;
+        g3.M = cols; g3.Nc = R; g3.K = nb;
+        g3.A = Wt2; g3.sA = wt_stride; g3.lda = W; g3.tA = 0;
+        g3.B = Vs; g3.sB = (long long)N * QRB; g3.ldb = N; g3.tB = 1;
+        g3.C = At; g3.sC = stride; g3.ldc = W; g3.tC = 0;
+        g3.alpha = -1.0; g3.beta = 1.0; g3.batch = batch;
+        e = launch_gemm<double>(g3, tile, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------------------------
+//  Blocked back-substitution X = R^{-1} Z for the row-major system (R in columns [0, N), Z in
+//  columns [c0, c0+P), solved in place), 32-row blocks from the bottom:
+//    trsm_diag_kernel  X_k = R_kk^{-1} Z_k, one thread per right-hand side, R_kk in LDS;
+//    GEMM (MFMA)       Z_{0:k} -= R_{0:k, k} X_k, as Z^T_{:, 0:k} -= X_k^T R_{0:k, k}^T.
+// ---------------------------------------------------------------------------------------------
+#define TRB 32
+
+__global__ __launch_bounds__(256) void trsm_diag_kernel(double* __restrict__ bufs, long long stride, int W, int c0,
+                                                        int P, int k0, int nbk)
+{
+    __shared__ double Rk[TRB][TRB + 1];
+    double* A = bufs + (long long)blockIdx.y * stride;
+    for (int e = threadIdx.x; e < TRB * TRB; e += 256) {
+        const int m = e / TRB, q = e % TRB;
+        Rk[m][q] = (m < nbk && q < nbk) ? A[(long long)(k0 + m) * W + k0 + q] : 0.0;
+    }
+    __syncthreads();
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= P) return;
+    double* Z = A + c0 + j;
+    double x[TRB];
+#pragma unroll
+    for (int m = 0; m < TRB; ++m) x[m] = (m < nbk) ? Z[(long long)(k0 + m) * W] : 0.0;
+#pragma unroll
+    for (int m = TRB - 1; m >= 0; --m) {
+        if (m < nbk) {
+            double v = x[m];
+#pragma unroll
+            for (int q = m + 1; q < TRB; ++q)
+                if (q < nbk) v -= Rk[m][q] * x[q];
+            x[m] = v / Rk[m][m];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < TRB; ++m)
+        if (m < nbk) Z[(long long)(k0 + m) * W] = x[m];
+}
+
+hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,
+                               hipStream_t s)
+{
+    if (N < 1 || P < 1 || c0 < N || c0 + P > W) return hipErrorInvalidValue;
+    const int nblk = (N + TRB - 1) / TRB;
+    for (int kb = nblk - 1; kb >= 0; --kb) {
+        const int k0 = kb * TRB, nbk = (N - k0 < TRB) ? N - k0 : TRB;
+        hipLaunchKernelGGL(trsm_diag_kernel, dim3((P + 255) / 256, batch), dim3(256), 0, s, buf, stride, W, c0, P, k0,
+                           nbk);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        if (k0 == 0) break;
+        GemmArgs<double> g;
+        g.M = P; g.Nc = k0; g.K = nbk;
+        g.A = buf + c0 + (long long)k0 * W; g.sA = stride; g.lda = W; g.tA = 0;
+        g.B = buf + k0; g.sB = stride; g.ldb = W; g.tB = 0;
+        g.C = buf + c0; g.sC = stride; g.ldc = W; g.tC = 0;
+        g.alpha = -1.0; g.beta = 1.0; g.batch = batch;
+        e = launch_gemm<double>(g, tile, s);
         if (e != hipSuccess) return e;
     }
     return hipSuccess;

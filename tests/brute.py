@@ -86,3 +86,34 @@ def square_lattice_bands(nl):
     """a_k = 2 (cos kx + cos ky): eigenvalues of the adjacency (periodic images counted)."""
     ks = [2 * math.pi * m / nl for m in range(nl)]
     return np.array([2.0 * (math.cos(kx) + math.cos(ky)) for kx in ks for ky in ks])
+
+
+def displaced_mp(expk, alpha, fields, dps=100):
+    """Time-displaced Green's functions at `dps` digits on the FP64 inputs; fields is [L][N].
+
+    Returns (gt0, g0t), each [L][N][N] in the reference's storage order (src/dqmc_core.cpp:343-382):
+    index t >= 1 holds G(t,0) = B(t)...B(1) G(0) and G(0,t) = -(1 - G(0)) (B(t)...B(1))^{-1};
+    index 0 holds G(0,0) and G(0,0) - 1 (the tau -> 0 conventions)."""
+    L, n = fields.shape
+    with mp.workdps(dps):
+        E = _to_mp(expk)
+        bs = []
+        for l in range(1, L + 1):
+            d = [mp.e ** (mp.mpf(alpha) * int(fields[l % L][i])) for i in range(n)]
+            Bl = E.copy()
+            for i in range(n):
+                for j in range(n):
+                    Bl[i, j] *= d[i]
+            bs.append(Bl)
+        M = mp.eye(n)
+        for Bl in bs:
+            M = Bl * M
+        G0 = (mp.eye(n) + M) ** -1
+        gt0 = [_to_np(G0)]
+        g0t = [_to_np(G0 - mp.eye(n))]
+        P = mp.eye(n)
+        for t in range(1, L):
+            P = bs[t - 1] * P
+            gt0.append(_to_np(P * G0))
+            g0t.append(_to_np(-(mp.eye(n) - G0) * P ** -1))
+        return np.array(gt0), np.array(g0t)

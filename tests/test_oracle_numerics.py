@@ -91,6 +91,23 @@ def test_equaltime_gf_vs_mpmath(t):
     assert np.abs(g - gm).max() < 1e-10
 
 
+def test_dynamic_gf_vs_mpmath():
+    # G(t,0) = B(t)..B(1) G(0) and G(0,t) = -(1 - G(0)) (B(t)..B(1))^{-1} for every stored slice,
+    # including the tau -> 0 conventions at index 0, from a dynamic sweep with 3 stabilizations.
+    nl, nt = 3, 12
+    c = orc.Chain(nl, nt, 1.5, 4.0, 0.2, pace=5, eqtime=True, dynamic=True)
+    c.set_seed(23)
+    c.fields_random()
+    assert c.init() == 0
+    c.set_thermalization(False)
+    assert c.sweep_dynamic() == 0
+    e, _ = c.expk()
+    gt0, g0t = brute.displaced_mp(e, c.alpha, c.fields, dps=60)
+    for t in range(nt):
+        assert np.abs(c.gf(2, t) - gt0[t]).max() < 1e-11, t
+        assert np.abs(c.gf(3, t) - g0t[t]).max() < 1e-11, t
+
+
 def test_init_gf_example_fields_vs_mpmath():
     # The reference's own field fixture (example/ising.fields: nl=4, nt=160, beta=8, u=4, mu=0).
     rows = np.loadtxt(os.path.join(GOLD, "example", "ising.fields"), skiprows=1, dtype=int)
@@ -171,3 +188,38 @@ def test_atomic_limit_double_occupancy():
     want = math.exp(beta * u / 4) / (2 * math.exp(beta * u / 4) + 2 * math.exp(-beta * u / 4))
     err = acc.std() / math.sqrt(len(acc) / 20)   # generous autocorrelation allowance
     assert abs(acc.mean() - want) < 5 * err + 1e-3
+
+
+def test_observables_u0_closed_form():
+    # U = 0: G(t) = (1 + e^{-beta K})^{-1} for every slice and G(t,0) = e^{-tau K} G, so with the
+    # Fermi factor f_k of eps_k = -2 nnt (cos kx + cos ky) - mu every estimator of
+    # include/observable_methods.hpp has a closed form:
+    #   filling 2 mean f, double occupancy (mean f)^2, s-wave pairing mean(f^2 + (1-f)^2),
+    #   G(k, tau) = e^{-tau eps_k} (1 - f_k), LDOS mean_q e^{-tau eps_q} (1 - f_q),
+    #   transverse spin mean_q(e^{tau eps} f) * mean_q(e^{-tau eps} (1 - f)).
+    nl, nt, beta, nnt, mu = 4, 16, 2.0, 1.0, -0.4
+    c = orc.Chain(nl, nt, beta, 0.0, mu, nnt=nnt, pace=4, eqtime=True, dynamic=True)
+    c.set_seed(3)
+    c.fields_random()
+    assert c.init() == 0
+    c.set_thermalization(False)
+    assert c.sweep_dynamic() == 0
+    ks = [2 * math.pi * m / nl for m in range(nl)]
+    eps = np.array([-2 * nnt * (math.cos(kx) + math.cos(ky)) - mu for kx in ks for ky in ks])
+    f = 1.0 / (1.0 + np.exp(beta * eps))
+    fill, docc, swave = c.measure_equaltime()
+    assert abs(fill - 2 * f.mean()) < 1e-12
+    assert abs(docc - f.mean() ** 2) < 1e-12
+    assert abs(swave - np.mean(f ** 2 + (1 - f) ** 2)) < 1e-12
+    kst = np.array([(i / nl * 2 * math.pi - math.pi, j / nl * 2 * math.pi - math.pi)
+                    for i in range(math.ceil(nl / 2), nl + 1) for j in range(math.ceil(nl / 2), nl + 1)])
+    gk, ldos, spin = c.measure_dynamic(kst)
+    tau = np.arange(nt) * beta / nt
+    for q, (kx, ky) in enumerate(kst):
+        ek = -2 * nnt * (math.cos(kx) + math.cos(ky)) - mu
+        fk = 1.0 / (1.0 + math.exp(beta * ek))
+        assert np.abs(gk[q] - np.exp(-tau * ek) * (1 - fk)).max() < 1e-11
+    want_ldos = np.array([np.mean(np.exp(-t * eps) * (1 - f)) for t in tau])
+    want_spin = np.array([np.mean(np.exp(t * eps) * f) * np.mean(np.exp(-t * eps) * (1 - f)) for t in tau])
+    assert np.abs(ldos - want_ldos).max() < 1e-11
+    assert np.abs(spin - want_spin).max() < 1e-11
