@@ -70,6 +70,7 @@ struct dqmc_ctx {
     // lattice vector) instead of dense N x N GEMMs (DQMC_WRAP=dense selects the GEMMs)
     bool kron = true;
     bool trsm_v1 = false;   // DQMC_TRSM=v1: one-kernel thread-per-column back-substitution
+    int metro_threads = 1024;  // Metropolis workgroup size at N = 256 (DQMC_METRO_THREADS)
     double *qrV = nullptr, *qrT = nullptr, *qrWt = nullptr;   // blocked-QR scratch: V, T, Wt and Wt T
     int current_t = 0;
     long long updating_sweeps = 0;
@@ -361,14 +362,14 @@ int metropolis(dqmc_ctx* c, int t)
         a.fields = c->fields; a.N = c->N; a.L = c->L; a.t = t;
         a.rng = c->rng; a.powtab = c->powtab; a.delta = c->delta;
         a.accmask = c->accmask; a.acc_count = c->acc_count; a.kmax = c->kmax;
-        TIMED(c, KT_METRO, launch_metropolis<float>(a, c->B, c->stream));
+        TIMED(c, KT_METRO, launch_metropolis<float>(a, c->B, c->metro_threads, c->stream));
     } else {
         MetroArgs<double> a;
         a.G = c->G64; a.sG = c->NN();
         a.fields = c->fields; a.N = c->N; a.L = c->L; a.t = t;
         a.rng = c->rng; a.powtab = c->powtab; a.delta = c->delta;
         a.accmask = c->accmask; a.acc_count = c->acc_count; a.kmax = c->kmax;
-        TIMED(c, KT_METRO, launch_metropolis<double>(a, c->B, c->stream));
+        TIMED(c, KT_METRO, launch_metropolis<double>(a, c->B, c->metro_threads, c->stream));
     }
     return 0;
 }
@@ -772,6 +773,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         c->experimental = ex && ex[0] == '1';
         const char* q = std::getenv("DQMC_QR");
         c->qr_mode = (q && std::string(q) == "v1") ? 1 : 0;
+        const char* mt = std::getenv("DQMC_METRO_THREADS");
+        if (mt) c->metro_threads = std::atoi(mt);
         const char* tr = std::getenv("DQMC_TRSM");
         c->trsm_v1 = tr && std::string(tr) == "v1";
         const char* w = std::getenv("DQMC_WRAP");

@@ -79,8 +79,10 @@ struct MetroArgs {
     unsigned long long* acc_count;    // [chain]
     int kmax;
 };
+// threads: workgroup size for N >= 256 (256, 512 or 1024); smaller lattices use 2 x N rounded
+// up to a wave.
 template <typename T>
-hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, hipStream_t s);
+hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, int threads, hipStream_t s);
 // LDS bytes of one Metropolis workgroup (krows pending-update rows of X and Y).
 size_t metropolis_lds_bytes(int N, int krows, size_t esz);
 // One-time kernel attribute setup (large dynamic LDS); call before any graph capture.

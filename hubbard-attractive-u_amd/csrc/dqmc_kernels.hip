@@ -219,19 +219,32 @@ __global__ __launch_bounds__(256) void kron_left_kernel(KronArgs<T> a)
     load_efrag<T>(a.E, lane, ef, ef2);
     const T* X = a.X + (long long)b * a.sX;
     T* C = a.C + (long long)b * a.sC;
+    const int cbase = blockIdx.x * 16 + wave * 4;
+    // all four lattice vectors of the wave are loaded before the first MFMA
+    T vin[4][4];
+#pragma unroll
     for (int cc = 0; cc < 4; ++cc) {
-        const int c = blockIdx.x * 16 + wave * 4 + cc;
+        const int c = cbase + cc;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int xp = (lane >> 4) + 4 * s, yp = lane & 15;
+            T v = T(0);
+            if (c < N && xp < nl && yp < nl) {
+                const int i = xp + nl * yp;
+                v = X[(long long)c * N + i];
+                if (a.innd.mode != DIAG_NONE) v = T((double)v * diag_value(a.innd, b, i));
+            }
+            vin[cc][s] = v;
+        }
+    }
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        const int c = cbase + cc;
         if (c >= N) break;
         const double cs = (a.cold.mode != DIAG_NONE) ? diag_value(a.cold, b, c) : 1.0;
-        const T* xc = X + (long long)c * N;
         T* cc_out = C + (long long)c * N;
         kron_tile<T>(ef, ef2, lane, nl,
-            [&](int xp, int yp) {
-                const int i = xp + nl * yp;
-                T v = xc[i];
-                if (a.innd.mode != DIAG_NONE) v = T((double)v * diag_value(a.innd, b, i));
-                return v;
-            },
+            [&](int xp, int yp) { return vin[cc][(xp - (lane >> 4)) >> 2]; },
             [&](int x, int y, T z) {
                 const int i = x + nl * y;
                 double v = (double)z * cs;
@@ -396,10 +409,10 @@ size_t metropolis_lds_bytes(int N, int kb, size_t esz)
 // G(ri, cj) += sum_m X[ri][m] Y[m][cj] over the whole N x N matrix, on MFMA.  The tile is
 // computed transposed (MFMA rows = G columns) so that each lane group touches contiguous
 // column segments of the column-major G.  Each wave keeps FT tiles in flight (all loads of a
-// group are issued before its first MFMA) so that the global-memory latency overlaps.
-#define FT 8
-template <typename T>
-__device__ void flush_mfma(T* __restrict__ G, int N, const T* __restrict__ Xl, const T* __restrict__ Yl, int a4)
+// group are issued before its first MFMA) so that the global-memory latency overlaps; FT is 8,
+// or 4 under the 128-register budget of a 1024-thread workgroup.
+template <typename T, int FT>
+__device__ __forceinline__ void flush_mfma(T* __restrict__ G, int N, const T* __restrict__ Xl, const T* __restrict__ Yl, int a4)
 {
     typedef typename Mfma<T>::acc_t acc_t;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -453,8 +466,8 @@ __device__ void flush_mfma(T* __restrict__ G, int N, const T* __restrict__ Xl, c
 // columns/rows of G0 = G at block start; the block-restricted entries they need are the ones
 // recorded in XS / YS while the block was processed.  With kb = 1 every accepted flip is
 // applied immediately, as in the reference.
-template <typename T>
-__global__ __launch_bounds__(256) void metropolis_kernel(MetroArgs<T> a)
+template <typename T, int MT>
+__global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT / 256))) void metropolis_kernel(MetroArgs<T> a)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int N = a.N, kb = a.kmax;
@@ -518,38 +531,40 @@ __global__ __launch_bounds__(256) void metropolis_kernel(MetroArgs<T> a)
         }
         nacc += na;
         if (na == 0) continue;
-        // rebuild X (rows of G) and Y (columns of G) of the rank-na update from G0
+        // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  The two recursions
+        // are independent: with at least 2N threads one half builds X and the other half Y.
+        // Thread j keeps its column j of X / Y in Xl[m][j] / Yl[m][j] (in place, no barrier:
+        // no other thread touches column j of that array).  The Y threads read the block's
+        // rows G0(S, j) as one contiguous segment into Yl[c][j] first; row m of Y overwrites
+        // entry c = m only after every later accepted site (block index > m) has been read.
         const int a4 = ((na + 3) / 4) * 4;
-        for (int j = tid; j < N; j += nth) {
-            double xr[MB_MAX], yr[MB_MAX];
-#pragma unroll
-            for (int m = 0; m < MB_MAX; ++m) {
-                if (m < na) {
-                    const int im = b0 + L.ql[m];
-                    xr[m] = (double)G[j + (long long)im * N];
-                    yr[m] = (double)G[im + (long long)j * N] - (j == im ? 1.0 : 0.0);
-                }
-            }
-#pragma unroll
-            for (int m = 0; m < MB_MAX; ++m) {
-                if (m < na) {
+        const bool split = nth >= 2 * N;
+        for (int e = tid; e < (split ? 2 * N : N); e += nth) {
+            const int j = split ? (e % N) : e;
+            const bool do_x = !split || e < N, do_y = !split || e >= N;
+            if (do_x) {
+                for (int m = 0; m < na; ++m) L.Xl[m * N + j] = G[j + (long long)(b0 + L.ql[m]) * N];
+                for (int m = 0; m < na; ++m) {
                     const int qm = L.ql[m];
-                    double sx = xr[m], sy = yr[m];
-#pragma unroll
-                    for (int mp = 0; mp < m; ++mp) {
-                        sx += xr[mp] * L.YS[mp * kb + qm];
-                        sy += L.XS[mp * kb + qm] * yr[mp];
-                    }
-                    xr[m] = L.gam[m] * sx;
-                    yr[m] = sy;
-                    L.Xl[m * N + j] = T(xr[m]);
-                    L.Yl[m * N + j] = T(yr[m]);
+                    double sx = (double)L.Xl[m * N + j];
+                    for (int mp = 0; mp < m; ++mp) sx += (double)L.Xl[mp * N + j] * L.YS[mp * kb + qm];
+                    L.Xl[m * N + j] = T(L.gam[m] * sx);
                 }
+                for (int m = na; m < a4; ++m) L.Xl[m * N + j] = T(0);
             }
-            for (int m = na; m < a4; ++m) { L.Xl[m * N + j] = T(0); L.Yl[m * N + j] = T(0); }
+            if (do_y) {
+                for (int c = 0; c < bs; ++c) L.Yl[c * N + j] = G[(b0 + c) + (long long)j * N];
+                for (int m = 0; m < na; ++m) {
+                    const int qm = L.ql[m];
+                    double sy = (double)L.Yl[qm * N + j] - (j == b0 + qm ? 1.0 : 0.0);
+                    for (int mp = 0; mp < m; ++mp) sy += L.XS[mp * kb + qm] * (double)L.Yl[mp * N + j];
+                    L.Yl[m * N + j] = T(sy);
+                }
+                for (int m = na; m < a4; ++m) L.Yl[m * N + j] = T(0);
+            }
         }
         __syncthreads();
-        flush_mfma<T>(G, N, L.Xl, L.Yl, a4);
+        flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, a4);
         __syncthreads();
     }
     __syncthreads();
@@ -565,29 +580,35 @@ __global__ void qr_kernel(double* __restrict__ bufs, long long stride, int N, in
 
 hipError_t prepare_kernels(bool blocked_qr)
 {
-    hipError_t e = hipFuncSetAttribute((const void*)metropolis_kernel<double>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return e;
-    if (blocked_qr) {
-        e = hipFuncSetAttribute((const void*)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    const void* ks[] = {(const void*)metropolis_kernel<double, 256>, (const void*)metropolis_kernel<double, 512>,
+                        (const void*)metropolis_kernel<double, 1024>, (const void*)metropolis_kernel<float, 256>,
+                        (const void*)metropolis_kernel<float, 512>, (const void*)metropolis_kernel<float, 1024>};
+    for (const void* k : ks) {
+        hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
     }
-    return hipFuncSetAttribute((const void*)metropolis_kernel<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               160 * 1024);
+    if (blocked_qr)
+        return hipFuncSetAttribute((const void*)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    return hipSuccess;
 }
 
 template <typename T>
-hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, hipStream_t s)
+hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, int threads, hipStream_t s)
 {
     if (a.kmax < 1 || a.kmax > MB_MAX) return hipErrorInvalidValue;
     const size_t shm = metropolis_lds_bytes(a.N, a.kmax, sizeof(T));
     if (shm > 160 * 1024) return hipErrorInvalidValue;
-    const int threads = a.N >= 256 ? 256 : ((a.N + 63) / 64) * 64;
-    hipLaunchKernelGGL(metropolis_kernel<T>, dim3(batch), dim3(threads), shm, s, a);
+    if (a.N < 256) threads = ((a.N + 63) / 64) * 64 * 2 <= 1024 ? ((a.N + 63) / 64) * 64 * 2 : 1024;
+    if (threads <= 256)
+        hipLaunchKernelGGL((metropolis_kernel<T, 256>), dim3(batch), dim3(threads < 64 ? 64 : threads), shm, s, a);
+    else if (threads <= 512)
+        hipLaunchKernelGGL((metropolis_kernel<T, 512>), dim3(batch), dim3(threads), shm, s, a);
+    else
+        hipLaunchKernelGGL((metropolis_kernel<T, 1024>), dim3(batch), dim3(1024), shm, s, a);
     return hipGetLastError();
 }
-template hipError_t launch_metropolis<double>(const MetroArgs<double>&, int, hipStream_t);
-template hipError_t launch_metropolis<float>(const MetroArgs<float>&, int, hipStream_t);
+template hipError_t launch_metropolis<double>(const MetroArgs<double>&, int, int, hipStream_t);
+template hipError_t launch_metropolis<float>(const MetroArgs<float>&, int, int, hipStream_t);
 
 // ---------------------------------------------------------------------------------------------
 //  Blocked Householder QR of a row-major N x N block with Q^T applied to W-N trailing columns.
