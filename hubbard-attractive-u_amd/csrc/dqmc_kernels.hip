@@ -532,35 +532,53 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
         nacc += na;
         if (na == 0) continue;
         // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  The two recursions
-        // are independent: with at least 2N threads one half builds X and the other half Y.
-        // Thread j keeps its column j of X / Y in Xl[m][j] / Yl[m][j] (in place, no barrier:
-        // no other thread touches column j of that array).  The Y threads read the block's
-        // rows G0(S, j) as one contiguous segment into Yl[c][j] first; row m of Y overwrites
-        // entry c = m only after every later accepted site (block index > m) has been read.
+        // are independent: with at least 2N threads one half builds X and the other half Y,
+        // each in registers with the recursion unrolled.  The Y threads read the block's rows
+        // G0(S, j) as one contiguous segment (staged in their own column j of Yl) instead of na
+        // scattered row reads.
         const int a4 = ((na + 3) / 4) * 4;
         const bool split = nth >= 2 * N;
         for (int e = tid; e < (split ? 2 * N : N); e += nth) {
             const int j = split ? (e % N) : e;
             const bool do_x = !split || e < N, do_y = !split || e >= N;
             if (do_x) {
-                for (int m = 0; m < na; ++m) L.Xl[m * N + j] = G[j + (long long)(b0 + L.ql[m]) * N];
-                for (int m = 0; m < na; ++m) {
-                    const int qm = L.ql[m];
-                    double sx = (double)L.Xl[m * N + j];
-                    for (int mp = 0; mp < m; ++mp) sx += (double)L.Xl[mp * N + j] * L.YS[mp * kb + qm];
-                    L.Xl[m * N + j] = T(L.gam[m] * sx);
+                double xr[MB_MAX];
+#pragma unroll
+                for (int m = 0; m < MB_MAX; ++m)
+                    xr[m] = (m < na) ? (double)G[j + (long long)(b0 + L.ql[m]) * N] : 0.0;
+#pragma unroll
+                for (int m = 0; m < MB_MAX; ++m) {
+                    if (m < na) {
+                        const int qm = L.ql[m];
+                        double sx = xr[m];
+#pragma unroll
+                        for (int mp = 0; mp < m; ++mp) sx += xr[mp] * L.YS[mp * kb + qm];
+                        xr[m] = L.gam[m] * sx;
+                    }
                 }
-                for (int m = na; m < a4; ++m) L.Xl[m * N + j] = T(0);
+#pragma unroll
+                for (int m = 0; m < MB_MAX; ++m)
+                    if (m < a4) L.Xl[m * N + j] = T(xr[m]);
             }
             if (do_y) {
                 for (int c = 0; c < bs; ++c) L.Yl[c * N + j] = G[(b0 + c) + (long long)j * N];
-                for (int m = 0; m < na; ++m) {
-                    const int qm = L.ql[m];
-                    double sy = (double)L.Yl[qm * N + j] - (j == b0 + qm ? 1.0 : 0.0);
-                    for (int mp = 0; mp < m; ++mp) sy += L.XS[mp * kb + qm] * (double)L.Yl[mp * N + j];
-                    L.Yl[m * N + j] = T(sy);
+                double yr[MB_MAX];
+#pragma unroll
+                for (int m = 0; m < MB_MAX; ++m)
+                    yr[m] = (m < na) ? (double)L.Yl[L.ql[m] * N + j] - (j == b0 + L.ql[m] ? 1.0 : 0.0) : 0.0;
+#pragma unroll
+                for (int m = 0; m < MB_MAX; ++m) {
+                    if (m < na) {
+                        const int qm = L.ql[m];
+                        double sy = yr[m];
+#pragma unroll
+                        for (int mp = 0; mp < m; ++mp) sy += L.XS[mp * kb + qm] * yr[mp];
+                        yr[m] = sy;
+                    }
                 }
-                for (int m = na; m < a4; ++m) L.Yl[m * N + j] = T(0);
+#pragma unroll
+                for (int m = 0; m < MB_MAX; ++m)
+                    if (m < a4) L.Yl[m * N + j] = T(yr[m]);
             }
         }
         __syncthreads();
