@@ -68,6 +68,20 @@ def cpu_baseline(seconds):
                       (cores, "MKL" if res[0][2] else "in-repo C", sweeps, wall)}
 
 
+def measured_traffic():
+    """HBM bytes per launch of the Metropolis kernel on this workload, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE x fetch scale + WRITE_SIZE x write scale, the scales
+    from a device copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "metropolis_kernel_n256_b256_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return {"traffic": int(line.split(":")[1].strip()), "traffic_source": "profiles/" + os.path.basename(path)}
+    except OSError:
+        pass
+    return {}
+
+
 # ----------------------------------------------------------------------------- GPU legs
 def make_engine(gpu, chains, precision, k):
     e = gpu.Engine(NL, NT, BETA, U, mu=MU, pace=PACE, n_chains=chains, precision=precision, delay_k=k)
@@ -107,6 +121,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 (1 chain, k=1) leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision check leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one device copy of 256 MiB (the reference kernel for rocprofv3 --pmc scales)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported "
@@ -117,6 +135,15 @@ def main():
 
     import dqmc_amd as gpu
     log = lambda *a: print(*a, file=sys.stderr, flush=True)
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(32 << 20, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] calibration copy: %d bytes read, %d bytes written" % (src.numel() * 8, dst.numel() * 8))
+        del src, dst
 
     # headline: FP64 batched, delayed updates
     log("[bench] batched FP64: chains=%d k=%d" % (args.chains, args.delay))
@@ -127,13 +154,16 @@ def main():
     acc = sum(e.stats(c)["accepts"] for c in range(args.chains)) / float(sum(e.stats(c)["visits"] for c in range(args.chains)))
     log("[bench] value %.2f sweeps/s (%.3f s for %d steps)" % (value, dt, args.steps))
 
-    # roofline: live per-launch timing of the dominant kernel over extra timed sweeps
+    # roofline: live per-launch timing (HIP events on the engine's stream) of the Metropolis
+    # kernel, the path's largest single kernel, over two extra sweeps; the other timer classes
+    # give the kernel-time shares
     blocked = os.environ.get("DQMC_EXPERIMENTAL_KERNELS", "") == "1"
     qr_name = "qr_kernel" if blocked else ("qr_v1_kernel" if os.environ.get("DQMC_QR", "") == "v1"
-                                           else "qr_blocked (qr_panel_kernel + 2 gemm_kernel<double> per 32 columns)")
-    kernel_names = {0: "metropolis_kernel<double>", 1: qr_name,
-                    2: "gemm_kernel<double>", 3: "trsm_kernel" if blocked else "trsm_v1_kernel",
-                    4: "kron_left/right_kernel<double>"}
+                                           else "blocked QR (qr_panel4_kernel + 3 gemm_kernel<double> per 32 columns)")
+    trsm_name = "trsm_kernel" if blocked else ("trsm_v1_kernel" if os.environ.get("DQMC_TRSM", "") == "v1"
+                                               else "blocked trsm (trsm_diag_kernel + gemm_kernel<double>)")
+    kernel_names = {0: "metropolis_kernel<double, 1024>", 1: qr_name, 2: "gemm_kernel<double> (stack/recompose)",
+                    3: trsm_name, 4: "kron_left/right_kernel<double>"}
     prof = {}
     for kid in (0, 1, 2, 3, 4):
         e.kernel_timer_begin(kid, 4096)
@@ -142,46 +172,40 @@ def main():
         ms, n = e.kernel_timer_end()
         a1 = sum(e.stats(c)["accepts"] for c in range(args.chains))
         prof[kid] = (ms, n, a1 - a0)
-    dom = max(prof, key=lambda k: prof[k][0])
     N = NL * NL
-    B = args.chains
-    ms, n, accs = prof[dom]
+    ms, n, accs = prof[0]
     avg_s = ms / n / 1e3
-    if dom == 0:
-        # rank-a updates: 2 N^2 flops per accepted flip, summed over the accepts of the launches
-        flops = 2.0 * N * N * accs / n
-        roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
-    elif dom == 1:
-        # Householder QR of N x N with Q^T applied to N right-hand sides: (10/3) N^3 per chain
-        flops = (10.0 / 3.0) * N ** 3 * B
-        roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
-    elif dom == 2:
-        flops = 2.0 * N ** 3 * B
-        roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
-    else:
-        flops = 1.0 * N ** 3 * B
-        roof = {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s"}
-    roof["achieved"] = round(roof["achieved"], 4)
+    # algorithmic work of one launch (one slice of every chain): the accepted flips' rank-1
+    # Sherman-Morrison updates, 2 N^2 flops each (src/hubbard.cpp:132-139)
+    flops = 2.0 * N * N * accs / n
+    roof = {"bound": "mfma", "achieved": round(flops / avg_s / 1e12, 4), "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s"}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 5)
     roof["traffic"] = None
-    roof["kernel"] = kernel_names[dom]
+    roof.update(measured_traffic())
+    roof["kernel"] = kernel_names[0]
     roof["avg_launch_us"] = round(avg_s * 1e6, 2)
-    roof["kernel_share"] = {kernel_names[k]: round(prof[k][0], 2) for k in prof}
+    roof["flops_per_launch"] = round(flops)
+    roof["kernel_time_ms_2_sweeps"] = {kernel_names[k]: round(prof[k][0], 2) for k in prof}
     e.close()
 
     # denominator: plain FP64 GPU form, 1 chain, rank-1 updates
-    log("[bench] plain FP64: 1 chain k=1")
-    p = make_engine(gpu, 1, gpu.FP64, 1)
-    plain, _ = timed(p, 1, args.plain_steps, 1, not args.no_graphs)
-    plain_dG = p.stats(0)["max_abs_dG"]
-    p.close()
+    plain = plain_dG = None
+    if not args.no_plain:
+        log("[bench] plain FP64: 1 chain k=1")
+        p = make_engine(gpu, 1, gpu.FP64, 1)
+        plain, _ = timed(p, 1, args.plain_steps, 1, not args.no_graphs)
+        plain_dG = p.stats(0)["max_abs_dG"]
+        p.close()
 
     # mixed FP32 wraps / flushes: |dG| on a short run (reported; fails the 1e-6 bound here)
-    log("[bench] mixed FP32 check: 8 chains")
-    m = make_engine(gpu, 8, gpu.MIXED, args.delay)
-    mixed_rate, _ = timed(m, 8, 2, 0, False)
-    mixed_dG = max(m.stats(c)["max_abs_dG"] for c in range(8))
-    m.close()
+    mixed_rate = mixed_dG = None
+    if not args.no_mixed:
+        log("[bench] mixed FP32 check: 8 chains")
+        m = make_engine(gpu, 8, gpu.MIXED, args.delay)
+        mixed_rate, _ = timed(m, 8, 2, 0, False)
+        mixed_dG = max(m.stats(c)["max_abs_dG"] for c in range(8))
+        m.close()
 
     out = {
         "metric": "aggregate sweeps/s at N=256, L=80: mixed+batched vs FP64 GPU, |dG|<=1e-6",
@@ -201,11 +225,11 @@ def main():
         "max_abs_dG": dG,
         "dG_ok": dG <= 1e-6,
         "acceptance": round(acc, 4),
-        "plain_fp64_sweeps_per_s": round(plain, 4),
+        "plain_fp64_sweeps_per_s": round(plain, 4) if plain else None,
         "plain_fp64_max_abs_dG": plain_dG,
-        "speedup_vs_plain_fp64": round(value / plain, 2),
+        "speedup_vs_plain_fp64": round(value / plain, 2) if plain else None,
         "mixed_fp32_max_abs_dG": mixed_dG,
-        "mixed_fp32_sweeps_per_s_8chains": round(mixed_rate, 3),
+        "mixed_fp32_sweeps_per_s_8chains": round(mixed_rate, 3) if mixed_rate else None,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
