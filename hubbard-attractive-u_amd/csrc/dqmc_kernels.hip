@@ -83,10 +83,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> g)
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
+    // The next K tile is fetched into registers while the current one is multiplied from LDS.
+    constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+    T ra[NA], rb[NB];
+    auto fetch = [&](int k0) {
 #pragma unroll
-        for (int e0 = 0; e0 < BM * BK; e0 += 256) {
-            const int e = e0 + tid;
+        for (int u = 0; u < NA; ++u) {
+            const int e = u * 256 + tid;
             int i, kk;
             if (!g.tA) { i = e % BM; kk = e / BM; } else { kk = e % BK; i = e / BK; }
             const int gi = m0 + i, gk = k0 + kk;
@@ -95,19 +98,37 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> g)
                 v = g.tA ? A[gk + (long long)gi * g.lda] : A[gi + (long long)gk * g.lda];
                 if (g.innd.mode != DIAG_NONE) v = T((double)v * diag_value(g.innd, b, gk));
             }
-            As[kk][i] = v;
+            ra[u] = v;
         }
 #pragma unroll
-        for (int e0 = 0; e0 < BN * BK; e0 += 256) {
-            const int e = e0 + tid;
+        for (int u = 0; u < NB; ++u) {
+            const int e = u * 256 + tid;
             int j, kk;
             if (!g.tB) { kk = e % BK; j = e / BK; } else { j = e % BN; kk = e / BN; }
             const int gj = n0 + j, gk = k0 + kk;
             T v = T(0);
             if (gj < g.Nc && gk < g.K) v = g.tB ? B[gj + (long long)gk * g.ldb] : B[gk + (long long)gj * g.ldb];
-            Bs[kk][j] = v;
+            rb[u] = v;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = u * 256 + tid;
+            int i, kk;
+            if (!g.tA) { i = e % BM; kk = e / BM; } else { kk = e % BK; i = e / BK; }
+            As[kk][i] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = u * 256 + tid;
+            int j, kk;
+            if (!g.tB) { kk = e % BK; j = e / BK; } else { j = e % BN; kk = e / BN; }
+            Bs[kk][j] = rb[u];
         }
         __syncthreads();
+        if (k0 + BK < g.K) fetch(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             T af[TM], bf[TN];
