@@ -241,6 +241,16 @@ __global__ __launch_bounds__(256) void kron_left_kernel(KronArgs<T> a)
     const T* X = a.X + (long long)b * a.sX;
     T* C = a.C + (long long)b * a.sC;
     const int cbase = blockIdx.x * 16 + wave * 4;
+    // The diagonal scalings depend on the lattice site only, so each lane evaluates the ones of
+    // its four input sites and four output sites once for all columns.
+    double din[4], dout[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int xp = (lane >> 4) + 4 * s, yp = lane & 15;
+        din[s] = (a.innd.mode != DIAG_NONE && xp < nl && yp < nl) ? diag_value(a.innd, b, xp + nl * yp) : 1.0;
+        const int x = kron_row<T>(lane, s), y = lane & 15;
+        dout[s] = (a.rowd.mode != DIAG_NONE && x < nl && y < nl) ? diag_value(a.rowd, b, x + nl * y) : 1.0;
+    }
     // all four lattice vectors of the wave are loaded before the first MFMA
     T vin[4][4];
 #pragma unroll
@@ -251,9 +261,8 @@ __global__ __launch_bounds__(256) void kron_left_kernel(KronArgs<T> a)
             const int xp = (lane >> 4) + 4 * s, yp = lane & 15;
             T v = T(0);
             if (c < N && xp < nl && yp < nl) {
-                const int i = xp + nl * yp;
-                v = X[(long long)c * N + i];
-                if (a.innd.mode != DIAG_NONE) v = T((double)v * diag_value(a.innd, b, i));
+                v = X[(long long)c * N + xp + nl * yp];
+                if (a.innd.mode != DIAG_NONE) v = T((double)v * din[s]);
             }
             vin[cc][s] = v;
         }
@@ -264,14 +273,18 @@ __global__ __launch_bounds__(256) void kron_left_kernel(KronArgs<T> a)
         if (c >= N) break;
         const double cs = (a.cold.mode != DIAG_NONE) ? diag_value(a.cold, b, c) : 1.0;
         T* cc_out = C + (long long)c * N;
-        kron_tile<T>(ef, ef2, lane, nl,
-            [&](int xp, int yp) { return vin[cc][(xp - (lane >> 4)) >> 2]; },
-            [&](int x, int y, T z) {
-                const int i = x + nl * y;
-                double v = (double)z * cs;
-                if (a.rowd.mode != DIAG_NONE) v *= diag_value(a.rowd, b, i);
-                cc_out[i] = T(v);
-            });
+        typedef typename Mfma<T>::acc_t acc_t;
+        acc_t y = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) y = Mfma<T>::mma(vin[cc][s], ef[s], y);
+        acc_t z = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) z = Mfma<T>::mma(y[s], ef2[s], z);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int x = kron_row<T>(lane, r), yy = lane & 15;
+            if (x < nl && yy < nl) cc_out[x + nl * yy] = T((double)z[r] * cs * dout[r]);
+        }
     }
 }
 
