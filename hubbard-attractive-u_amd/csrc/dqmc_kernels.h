@@ -100,8 +100,10 @@ hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, 
 // trailing columns.  Scratch per chain: Vs N*32 doubles, Ts 32*32 doubles, and Wt holding two
 // blocks of wt_stride >= W*32 doubles (batch * wt_stride apart).  `tile` is the GEMM tile of
 // the trailing update.
+// `fused`: one kernel per panel applies the whole block reflector to 32-column tiles staged in LDS
+// (Wt unused); otherwise three GEMM launches.
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
-                             double* Wt, long long wt_stride, int tile, hipStream_t s);
+                             double* Wt, long long wt_stride, int tile, bool fused, hipStream_t s);
 // Blocked in-place back-substitution of the same system: 32-row diagonal solves (thread per
 // right-hand side) and MFMA GEMM updates of the rows above (`tile` = GEMM tile).
 hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,

@@ -630,6 +630,10 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
 
 __global__ void qr_kernel(double* __restrict__ bufs, long long stride, int N, int W);
 
+__global__ void qr_update_kernel(double* __restrict__ bufs, long long stride, int N, int W, int p0, int nb,
+                                 const double* __restrict__ Vs, const double* __restrict__ Ts);
+size_t qr_update_lds_bytes();
+
 hipError_t prepare_kernels(bool blocked_qr)
 {
     const void* ks[] = {(const void*)metropolis_kernel<double, 256>, (const void*)metropolis_kernel<double, 512>,
@@ -639,6 +643,9 @@ hipError_t prepare_kernels(bool blocked_qr)
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
     }
+    hipError_t e = hipFuncSetAttribute((const void*)qr_update_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)qr_update_lds_bytes());
+    if (e != hipSuccess) return e;
     if (blocked_qr)
         return hipFuncSetAttribute((const void*)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     return hipSuccess;
@@ -1191,8 +1198,100 @@ __global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bu
     }
 }
 
+// Fused trailing update of one panel: every trailing column a of the panel rows (right-hand
+// sides included) becomes a - V T^T V^T a, i.e. a - V (a^T V T)^T.  One workgroup takes 32
+// trailing columns of one chain: the column tile At (R x 32) and V (R x nb) are staged in LDS once,
+// W = At^T V (32 x nb), W2 = W T and At -= V W2^T run on MFMA, and At is written back once.
+#define QRU_CB 32
+struct QrUpdLds {
+    double At[256][QRU_CB + 1];
+    double V[256][QRB + 1];
+    double W[QRU_CB][QRB + 1];
+    double W2[QRU_CB][QRB + 1];
+};
+
+__global__ __launch_bounds__(512) void qr_update_kernel(double* __restrict__ bufs, long long stride, int N, int W,
+                                                        int p0, int nb, const double* __restrict__ Vs,
+                                                        const double* __restrict__ Ts)
+{
+    extern __shared__ __align__(16) unsigned char qru_smem[];
+    QrUpdLds& S = *reinterpret_cast<QrUpdLds*>(qru_smem);
+    typedef Mfma<double>::acc_t acc_t;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.y;
+    const int R = N - p0, R4 = (R + 3) & ~3;
+    const int cols = W - (p0 + nb), c0 = blockIdx.x * QRU_CB;
+    double* A = bufs + (long long)b * stride;
+    const double* V = Vs + (long long)b * N * QRB;
+    const double* T = Ts + (long long)b * QRB * QRB;
+    const long long base = (long long)p0 * W + p0 + nb + c0;   // At(0, 0)
+
+    for (int e = tid; e < R4 * QRU_CB; e += 512) {
+        const int r = e / QRU_CB, c = e % QRU_CB;
+        S.At[r][c] = (r < R && c0 + c < cols) ? A[base + (long long)r * W + c] : 0.0;
+    }
+    for (int e = tid; e < R4 * QRB; e += 512) {
+        const int r = e % R4, m = e / R4;
+        S.V[r][m] = (r < R && m < nb) ? V[r + (long long)m * N] : 0.0;
+    }
+    __syncthreads();
+    // W (32 x nb) = At^T V: four 16 x 16 tiles, waves 0..3
+    if (wave < 4) {
+        const int it = wave >> 1, mt = wave & 1;
+        acc_t acc = acc_t{0, 0, 0, 0};
+        for (int k0 = 0; k0 < R4; k0 += 4) {
+            const double av = S.At[k0 + (lane >> 4)][it * 16 + (lane & 15)];   // At^T(i, k)
+            const double bv = S.V[k0 + (lane >> 4)][mt * 16 + (lane & 15)];    // V(k, m)
+            acc = Mfma<double>::mma(av, bv, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) S.W[it * 16 + Mfma<double>::row(lane, r)][mt * 16 + (lane & 15)] = acc[r];
+    }
+    __syncthreads();
+    // W2 = W T (T upper triangular, column-major 32 x 32, zero beyond nb)
+    if (wave < 4) {
+        const int it = wave >> 1, mt = wave & 1;
+        acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int k0 = 0; k0 < QRB; k0 += 4) {
+            const double av = S.W[it * 16 + (lane & 15)][k0 + (lane >> 4)];
+            const int kk = k0 + (lane >> 4), mm = mt * 16 + (lane & 15);
+            const double bv = (kk < nb && mm < nb) ? T[kk + (long long)mm * QRB] : 0.0;
+            acc = Mfma<double>::mma(av, bv, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) S.W2[it * 16 + Mfma<double>::row(lane, r)][mt * 16 + (lane & 15)] = acc[r];
+    }
+    __syncthreads();
+    // At (R x 32) -= V W2^T, tiles of 16 rows x 16 columns spread over the 8 waves
+    const int nrt = (R + 15) / 16;
+    for (int tile = wave; tile < nrt * 2; tile += 8) {
+        const int rt = tile >> 1, ct = tile & 1;
+        acc_t acc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = rt * 16 + Mfma<double>::row(lane, r);
+            acc[r] = (row < R4) ? S.At[row][ct * 16 + (lane & 15)] : 0.0;
+        }
+#pragma unroll
+        for (int k0 = 0; k0 < QRB; k0 += 4) {
+            const int row = rt * 16 + (lane & 15);
+            const double av = (row < R4) ? S.V[row][k0 + (lane >> 4)] : 0.0;        // V(row, k)
+            const double bv = -S.W2[ct * 16 + (lane & 15)][k0 + (lane >> 4)];         // -W2^T(k, col)
+            acc = Mfma<double>::mma(av, bv, acc);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = rt * 16 + Mfma<double>::row(lane, r), col = ct * 16 + (lane & 15);
+            if (row < R && c0 + col < cols) A[base + (long long)row * W + col] = acc[r];
+        }
+    }
+}
+
+size_t qr_update_lds_bytes() { return sizeof(QrUpdLds); }
+
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
-                             double* Wt, long long wt_stride, int tile, hipStream_t s)
+                             double* Wt, long long wt_stride, int tile, bool fused, hipStream_t s)
 {
     // per chain: Vs N*QRB, Ts QRB*QRB, Wt 2 * wt_stride (Wt and Wt T), wt_stride >= W*QRB
     if (N > 256 || N < 1 || W <= N) return hipErrorInvalidValue;
@@ -1205,6 +1304,13 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (cols <= 0) continue;
+        if (fused) {
+            hipLaunchKernelGGL(qr_update_kernel, dim3((cols + QRU_CB - 1) / QRU_CB, batch), dim3(512),
+                               sizeof(QrUpdLds), s, buf, stride, N, W, p0, nb, Vs, Ts);
+            e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            continue;
+        }
         double* At = buf + (p0 + nb) + (long long)p0 * W;   // At^T: cols x R, column-major, ld W
         // Wt = At^T V
         GemmArgs<double> g1;
