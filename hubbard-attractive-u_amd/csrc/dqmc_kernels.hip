@@ -43,6 +43,24 @@ template <> struct Mfma<float> {
     }
 };
 
+// Chain-major block order for batched launches with `per` tiles per chain on a 1-D grid.  The
+// dispatcher deals consecutive workgroups round-robin over the 8 XCDs, each with its own L2;
+// when the chain count is a multiple of 8, workgroup L is given chain 8 (L/8 / per) + L%8 and
+// tile (L/8) % per, so every tile of a chain runs on one XCD and the operands the tiles share
+// are fetched from HBM into one L2 only.  Otherwise the plain order chain L/per, tile L%per.
+__device__ __forceinline__ void chain_tile(int per, int batch, int& b, int& t)
+{
+    const int L = blockIdx.x;
+    if ((batch & 7) == 0) {
+        const int s = L >> 3;
+        b = (s / per) * 8 + (L & 7);
+        t = s % per;
+    } else {
+        b = L / per;
+        t = L % per;
+    }
+}
+
 __device__ __forceinline__ double diag_value(const DiagArg& d, int b, int i)
 {
     switch (d.mode) {
@@ -69,8 +87,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> g)
     __shared__ T Bs[BK][BN + PAD];
     typedef typename Mfma<T>::acc_t acc_t;
 
-    const int b = blockIdx.z;
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const int gm = (g.M + BM - 1) / BM, gn = (g.Nc + BN - 1) / BN;
+    int b, t;
+    chain_tile(gm * gn, g.batch, b, t);
+    const int m0 = (t % gm) * BM, n0 = (t / gm) * BN;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const T* __restrict__ A = g.A + (long long)b * g.sA;
@@ -166,8 +186,9 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> g)
 template <typename T, int BM, int BN>
 static hipError_t launch_gemm_tile(const GemmArgs<T>& a, hipStream_t s)
 {
-    dim3 grid((a.M + BM - 1) / BM, (a.Nc + BN - 1) / BN, a.batch);
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, 16>), grid, dim3(256), 0, s, a);
+    const long long blocks = (long long)((a.M + BM - 1) / BM) * ((a.Nc + BN - 1) / BN) * a.batch;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, 16>), dim3((unsigned)blocks), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -630,7 +651,7 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
 
 __global__ void qr_kernel(double* __restrict__ bufs, long long stride, int N, int W);
 
-__global__ void qr_update_kernel(double* __restrict__ bufs, long long stride, int N, int W, int p0, int nb,
+__global__ void qr_update_kernel(double* __restrict__ bufs, long long stride, int N, int W, int batch, int p0, int nb,
                                  const double* __restrict__ Vs, const double* __restrict__ Ts);
 size_t qr_update_lds_bytes();
 
@@ -1211,16 +1232,18 @@ struct QrUpdLds {
 };
 
 __global__ __launch_bounds__(512) void qr_update_kernel(double* __restrict__ bufs, long long stride, int N, int W,
-                                                        int p0, int nb, const double* __restrict__ Vs,
+                                                        int batch, int p0, int nb, const double* __restrict__ Vs,
                                                         const double* __restrict__ Ts)
 {
     extern __shared__ __align__(16) unsigned char qru_smem[];
     QrUpdLds& S = *reinterpret_cast<QrUpdLds*>(qru_smem);
     typedef Mfma<double>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int b = blockIdx.y;
     const int R = N - p0, R4 = (R + 3) & ~3;
-    const int cols = W - (p0 + nb), c0 = blockIdx.x * QRU_CB;
+    const int cols = W - (p0 + nb);
+    int b, cb;
+    chain_tile((cols + QRU_CB - 1) / QRU_CB, batch, b, cb);
+    const int c0 = cb * QRU_CB;
     double* A = bufs + (long long)b * stride;
     const double* V = Vs + (long long)b * N * QRB;
     const double* T = Ts + (long long)b * QRB * QRB;
@@ -1305,8 +1328,8 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
         if (e != hipSuccess) return e;
         if (cols <= 0) continue;
         if (fused) {
-            hipLaunchKernelGGL(qr_update_kernel, dim3((cols + QRU_CB - 1) / QRU_CB, batch), dim3(512),
-                               sizeof(QrUpdLds), s, buf, stride, N, W, p0, nb, Vs, Ts);
+            hipLaunchKernelGGL(qr_update_kernel, dim3(((cols + QRU_CB - 1) / QRU_CB) * batch), dim3(512),
+                               sizeof(QrUpdLds), s, buf, stride, N, W, batch, p0, nb, Vs, Ts);
             e = hipGetLastError();
             if (e != hipSuccess) return e;
             continue;
