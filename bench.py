@@ -159,7 +159,7 @@ def main():
     # give the kernel-time shares
     blocked = os.environ.get("DQMC_EXPERIMENTAL_KERNELS", "") == "1"
     qr_name = "qr_kernel" if blocked else ("qr_v1_kernel" if os.environ.get("DQMC_QR", "") == "v1"
-                                           else "blocked QR (qr_panel4_kernel + 3 gemm_kernel<double> per 32 columns)")
+                                           else "blocked QR (qr_panel4_kernel + qr_update_kernel per 32 columns)")
     trsm_name = "trsm_kernel" if blocked else ("trsm_v1_kernel" if os.environ.get("DQMC_TRSM", "") == "v1"
                                                else "blocked trsm (trsm_diag_kernel + gemm_kernel<double>)")
     kernel_names = {0: "metropolis_kernel<double, 1024>", 1: qr_name, 2: "gemm_kernel<double> (stack/recompose)",

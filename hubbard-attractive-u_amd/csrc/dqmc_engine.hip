@@ -70,7 +70,7 @@ struct dqmc_ctx {
     // lattice vector) instead of dense N x N GEMMs (DQMC_WRAP=dense selects the GEMMs)
     bool kron = true;
     bool trsm_v1 = false;   // DQMC_TRSM=v1: one-kernel thread-per-column back-substitution
-    bool qr_fused = true;     // fused QR trailing update (DQMC_QR_UPDATE=gemm: three GEMMs)
+    int qr_update = QR_UPDATE_REG;   // QR trailing update (DQMC_QR_UPDATE=gemm: three GEMMs)
     int metro_threads = 1024;  // Metropolis workgroup size at N = 256 (DQMC_METRO_THREADS)
     double *qrV = nullptr, *qrT = nullptr, *qrWt = nullptr;   // blocked-QR scratch: V, T, Wt and Wt T
     int current_t = 0;
@@ -251,7 +251,7 @@ hipError_t qr_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int 
 {
     if (c->experimental) return launch_qr(buf, stride, N, W, batch, s);
     if (c->qr_mode == 1) return launch_qr_v1(buf, stride, N, W, batch, s);
-    return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrT, c->qrWt, 3LL * N * 32, c->tile64, c->qr_fused, s);
+    return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrT, c->qrWt, 3LL * N * 32, c->tile64, c->qr_update, s);
 }
 hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int c0, int P, int batch, hipStream_t s)
 {
@@ -775,7 +775,7 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         const char* q = std::getenv("DQMC_QR");
         c->qr_mode = (q && std::string(q) == "v1") ? 1 : 0;
         const char* qu = std::getenv("DQMC_QR_UPDATE");
-        c->qr_fused = !(qu && std::string(qu) == "gemm");
+        if (qu && std::string(qu) == "gemm") c->qr_update = QR_UPDATE_GEMM;
         const char* mt = std::getenv("DQMC_METRO_THREADS");
         if (mt) c->metro_threads = std::atoi(mt);
         const char* tr = std::getenv("DQMC_TRSM");

@@ -96,14 +96,15 @@ hipError_t launch_qr(double* buf, long long stride, int N, int W, int batch, hip
 // Same contracts, reference-path implementations (see the kernel file).
 hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, hipStream_t s);
 // Blocked Householder QR of the same row-major N x W system (N <= 256): per panel of 32 columns
-// one factorization launch (one workgroup per chain) and three MFMA GEMM launches for the
-// trailing columns.  Scratch per chain: Vs N*32 doubles, Ts 32*32 doubles, and Wt holding two
-// blocks of wt_stride >= W*32 doubles (batch * wt_stride apart).  `tile` is the GEMM tile of
-// the trailing update.
-// `fused`: one kernel per panel applies the whole block reflector to 32-column tiles staged in LDS
-// (Wt unused); otherwise three GEMM launches.
+// one factorization launch (one workgroup per chain) and the trailing update of the remaining
+// columns.  Scratch per chain: Vs N*32 doubles, Ts 32*32 doubles, and Wt holding two blocks of
+// wt_stride >= W*32 doubles (batch * wt_stride apart).
+// `update`: QR_UPDATE_REG (default) applies the whole block reflector in one launch per panel with
+// each 32-column tile held in registers (the first Wt block holds a row-major copy of V);
+// QR_UPDATE_GEMM uses three MFMA GEMM launches with tile `tile` for the last one.
+enum QrUpdate : int { QR_UPDATE_GEMM = 0, QR_UPDATE_REG = 2 };
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
-                             double* Wt, long long wt_stride, int tile, bool fused, hipStream_t s);
+                             double* Wt, long long wt_stride, int tile, int update, hipStream_t s);
 // Blocked in-place back-substitution of the same system: 32-row diagonal solves (thread per
 // right-hand side) and MFMA GEMM updates of the rows above (`tile` = GEMM tile).
 hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,

@@ -651,9 +651,6 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
 
 __global__ void qr_kernel(double* __restrict__ bufs, long long stride, int N, int W);
 
-__global__ void qr_update_kernel(double* __restrict__ bufs, long long stride, int N, int W, int batch, int p0, int nb,
-                                 const double* __restrict__ Vs, const double* __restrict__ Ts);
-size_t qr_update_lds_bytes();
 
 hipError_t prepare_kernels(bool blocked_qr)
 {
@@ -664,9 +661,6 @@ hipError_t prepare_kernels(bool blocked_qr)
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
     }
-    hipError_t e = hipFuncSetAttribute((const void*)qr_update_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)qr_update_lds_bytes());
-    if (e != hipSuccess) return e;
     if (blocked_qr)
         return hipFuncSetAttribute((const void*)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     return hipSuccess;
@@ -1084,7 +1078,8 @@ hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, 
 #define QCPT 8
 __global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bufs, long long stride, int N, int W,
                                                          int p0, int nb, double* __restrict__ Vs,
-                                                         double* __restrict__ Ts)
+                                                         double* __restrict__ Ts, double* __restrict__ Vrows,
+                                                         long long vr_stride)
 {
     __shared__ double red[16][QCPT];        // per-wave partial dot products
     __shared__ double red1[16];             // per-wave partial norms
@@ -1195,6 +1190,15 @@ __global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bu
                 V[r + (long long)col * N] = (r == col) ? 1.0 : (r > col ? x[m] : 0.0);
             }
         }
+        if (Vrows) {
+            // the same V row-major (row r at r * QRB), for row-fragment reads of the update
+            double* vr = Vrows + (long long)b * vr_stride + (long long)r * QRB;
+#pragma unroll
+            for (int m = 0; m < QCPT; ++m) {
+                const int col = g * QCPT + m;
+                if (col < nb) vr[col] = (r == col) ? 1.0 : (r > col ? x[m] : 0.0);
+            }
+        }
     }
     // T row j by thread j: T(j, j) = tau_j, T(j, i) = -tau_i sum_{k=j}^{i-1} T(j, k) VtV(k, i)
     if (t < QRB) {
@@ -1221,115 +1225,149 @@ __global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bu
 
 // Fused trailing update of one panel: every trailing column a of the panel rows (right-hand
 // sides included) becomes a - V T^T V^T a, i.e. a - V (a^T V T)^T.  One workgroup takes 32
-// trailing columns of one chain: the column tile At (R x 32) and V (R x nb) are staged in LDS once,
-// W = At^T V (32 x nb), W2 = W T and At -= V W2^T run on MFMA, and At is written back once.
-#define QRU_CB 32
-struct QrUpdLds {
-    double At[256][QRU_CB + 1];
-    double V[256][QRB + 1];
-    double W[QRU_CB][QRB + 1];
-    double W2[QRU_CB][QRB + 1];
-};
-
-__global__ __launch_bounds__(512) void qr_update_kernel(double* __restrict__ bufs, long long stride, int N, int W,
-                                                        int batch, int p0, int nb, const double* __restrict__ Vs,
-                                                        const double* __restrict__ Ts)
+// trailing columns of one chain and holds them in registers.  256 threads; wave w owns the
+// 16-row tiles rt = w, w + 4, w + 8, w + 12 of the 32-column tile At (R x 32), loaded straight
+// into f64 accumulator layout: item r of tile (rt, ct) holds At(16 rt + 4 r + l/16, 16 ct + l%16).
+// That is also the A-fragment of At^T for the k-step rows 16 rt + 4 r .. + 3, so
+//   1. each wave forms its partial W = At^T V (32 x 32) over its own rows, V read row-major;
+//   2. the four partials are summed in LDS in a fixed order (the only LDS traffic);
+//   3. every wave forms W2^T = T^T W^T, whose accumulator items are the B-fragments of step 4;
+//   4. At -= V W2^T on its own tiles (V read column-major: 16 consecutive rows per fragment),
+//      and stores them.
+// At is read and written exactly once; V and T come from L2.
+__global__ __launch_bounds__(256) void qr_update_kernel(double* __restrict__ bufs, long long stride, int N,
+                                                            int W, int batch, int p0, int nb,
+                                                            const double* __restrict__ Vs,
+                                                            const double* __restrict__ Vrows, long long vr_stride,
+                                                            const double* __restrict__ Ts)
 {
-    extern __shared__ __align__(16) unsigned char qru_smem[];
-    QrUpdLds& S = *reinterpret_cast<QrUpdLds*>(qru_smem);
+    __shared__ double Wp[4][QRB][QRB + 1];
     typedef Mfma<double>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int R = N - p0, R4 = (R + 3) & ~3;
-    const int cols = W - (p0 + nb);
+    const int lr = lane >> 4, lc = lane & 15;
+    const int R = N - p0, cols = W - (p0 + nb), nrt = (R + 15) >> 4;
     int b, cb;
-    chain_tile((cols + QRU_CB - 1) / QRU_CB, batch, b, cb);
-    const int c0 = cb * QRU_CB;
+    chain_tile((cols + QRB - 1) / QRB, batch, b, cb);
+    const int c0 = cb * QRB;
     double* A = bufs + (long long)b * stride;
     const double* V = Vs + (long long)b * N * QRB;
+    const double* Vr = Vrows + (long long)b * vr_stride;
     const double* T = Ts + (long long)b * QRB * QRB;
     const long long base = (long long)p0 * W + p0 + nb + c0;   // At(0, 0)
 
-    for (int e = tid; e < R4 * QRU_CB; e += 512) {
-        const int r = e / QRU_CB, c = e % QRU_CB;
-        S.At[r][c] = (r < R && c0 + c < cols) ? A[base + (long long)r * W + c] : 0.0;
-    }
-    for (int e = tid; e < R4 * QRB; e += 512) {
-        const int r = e % R4, m = e / R4;
-        S.V[r][m] = (r < R && m < nb) ? V[r + (long long)m * N] : 0.0;
-    }
-    __syncthreads();
-    // W (32 x nb) = At^T V: four 16 x 16 tiles, waves 0..3
-    if (wave < 4) {
-        const int it = wave >> 1, mt = wave & 1;
-        acc_t acc = acc_t{0, 0, 0, 0};
-        for (int k0 = 0; k0 < R4; k0 += 4) {
-            const double av = S.At[k0 + (lane >> 4)][it * 16 + (lane & 15)];   // At^T(i, k)
-            const double bv = S.V[k0 + (lane >> 4)][mt * 16 + (lane & 15)];    // V(k, m)
-            acc = Mfma<double>::mma(av, bv, acc);
-        }
+    acc_t at[4][2];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) S.W[it * 16 + Mfma<double>::row(lane, r)][mt * 16 + (lane & 15)] = acc[r];
-    }
-    __syncthreads();
-    // W2 = W T (T upper triangular, column-major 32 x 32, zero beyond nb)
-    if (wave < 4) {
-        const int it = wave >> 1, mt = wave & 1;
-        acc_t acc = acc_t{0, 0, 0, 0};
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-        for (int k0 = 0; k0 < QRB; k0 += 4) {
-            const double av = S.W[it * 16 + (lane & 15)][k0 + (lane >> 4)];
-            const int kk = k0 + (lane >> 4), mm = mt * 16 + (lane & 15);
-            const double bv = (kk < nb && mm < nb) ? T[kk + (long long)mm * QRB] : 0.0;
-            acc = Mfma<double>::mma(av, bv, acc);
-        }
+        for (int ct = 0; ct < 2; ++ct)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) S.W2[it * 16 + Mfma<double>::row(lane, r)][mt * 16 + (lane & 15)] = acc[r];
-    }
-    __syncthreads();
-    // At (R x 32) -= V W2^T, tiles of 16 rows x 16 columns spread over the 8 waves
-    const int nrt = (R + 15) / 16;
-    for (int tile = wave; tile < nrt * 2; tile += 8) {
-        const int rt = tile >> 1, ct = tile & 1;
-        acc_t acc;
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * (wave + 4 * q) + 4 * r + lr, col = 16 * ct + lc;
+                at[q][ct][r] = (row < R && c0 + col < cols) ? A[base + (long long)row * W + col] : 0.0;
+            }
+
+    // 1. partial W(i, m) = sum over this wave's rows k of At(k, i) V(k, m)
+    acc_t wp[2][2];
+#pragma unroll
+    for (int it = 0; it < 2; ++it)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) wp[it][mt] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (wave + 4 * q >= nrt) break;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int row = rt * 16 + Mfma<double>::row(lane, r);
-            acc[r] = (row < R4) ? S.At[row][ct * 16 + (lane & 15)] : 0.0;
-        }
+            const int k = 16 * (wave + 4 * q) + 4 * r + lr;
+            double bv[2];
 #pragma unroll
-        for (int k0 = 0; k0 < QRB; k0 += 4) {
-            const int row = rt * 16 + (lane & 15);
-            const double av = (row < R4) ? S.V[row][k0 + (lane >> 4)] : 0.0;        // V(row, k)
-            const double bv = -S.W2[ct * 16 + (lane & 15)][k0 + (lane >> 4)];         // -W2^T(k, col)
-            acc = Mfma<double>::mma(av, bv, acc);
-        }
+            for (int mt = 0; mt < 2; ++mt) {
+                const int m = 16 * mt + lc;
+                bv[mt] = (k < R && m < nb) ? Vr[(long long)k * QRB + m] : 0.0;
+            }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int row = rt * 16 + Mfma<double>::row(lane, r), col = ct * 16 + (lane & 15);
-            if (row < R && c0 + col < cols) A[base + (long long)row * W + col] = acc[r];
+            for (int it = 0; it < 2; ++it)
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) wp[it][mt] = Mfma<double>::mma(at[q][it][r], bv[mt], wp[it][mt]);
+        }
+    }
+    // 2. W = sum of the four partials, in wave order
+#pragma unroll
+    for (int it = 0; it < 2; ++it)
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Wp[wave][16 * it + 4 * r + lr][16 * mt + lc] = wp[it][mt][r];
+    __syncthreads();
+    for (int e = tid; e < QRB * QRB; e += 256) {
+        const int i = e >> 5, m = e & 31;
+        Wp[0][i][m] = ((Wp[0][i][m] + Wp[1][i][m]) + Wp[2][i][m]) + Wp[3][i][m];
+    }
+    __syncthreads();
+    // 3. W2^T(m, i) = sum_j T(j, m) W(i, j); item r of w2t[mt][it] is W2^T(16 mt + 4 r + l/16, 16 it + l%16)
+    acc_t w2t[2][2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+            for (int j0 = 0; j0 < QRB; j0 += 4) {
+                const int j = j0 + lr, m = 16 * mt + lc;
+                const double av = (j < nb && m < nb) ? T[j + (long long)m * QRB] : 0.0;
+                const double bv = Wp[0][16 * it + lc][j];
+                acc = Mfma<double>::mma(av, bv, acc);
+            }
+            w2t[mt][it] = -acc;
+        }
+    // 4. At -= V W2^T on this wave's tiles
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int rt = wave + 4 * q;
+        if (rt >= nrt) break;
+        double av[2][4];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * rt + lc, m = 16 * mt + 4 * r + lr;
+                av[mt][r] = (row < R && m < nb) ? V[row + (long long)m * N] : 0.0;
+            }
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            acc_t acc = at[q][ct];
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc = Mfma<double>::mma(av[mt][r], w2t[mt][ct][r], acc);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * rt + 4 * r + lr, col = 16 * ct + lc;
+                if (row < R && c0 + col < cols) A[base + (long long)row * W + col] = acc[r];
+            }
         }
     }
 }
 
-size_t qr_update_lds_bytes() { return sizeof(QrUpdLds); }
-
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
-                             double* Wt, long long wt_stride, int tile, bool fused, hipStream_t s)
+                             double* Wt, long long wt_stride, int tile, int update, hipStream_t s)
 {
-    // per chain: Vs N*QRB, Ts QRB*QRB, Wt 2 * wt_stride (Wt and Wt T), wt_stride >= W*QRB
+    // per chain: Vs N*QRB, Ts QRB*QRB, Wt 2 * wt_stride (Wt and Wt T; in QR_UPDATE_REG the first
+    // holds the row-major copy of V), wt_stride >= W*QRB
     if (N > 256 || N < 1 || W <= N) return hipErrorInvalidValue;
     if (wt_stride < (long long)W * QRB) return hipErrorInvalidValue;
     double* Wt2 = Wt + (long long)batch * wt_stride;
     for (int p0 = 0; p0 < N; p0 += QRB) {
         const int nb = (N - p0 < QRB) ? N - p0 : QRB;
         const int R = N - p0, cols = W - (p0 + nb);
-        hipLaunchKernelGGL(qr_panel4_kernel, dim3(batch), dim3(1024), 0, s, buf, stride, N, W, p0, nb, Vs, Ts);
+        const bool reg = update == QR_UPDATE_REG;
+        hipLaunchKernelGGL(qr_panel4_kernel, dim3(batch), dim3(1024), 0, s, buf, stride, N, W, p0, nb, Vs, Ts,
+                           reg ? Wt : nullptr, wt_stride);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (cols <= 0) continue;
-        if (fused) {
-            hipLaunchKernelGGL(qr_update_kernel, dim3(((cols + QRU_CB - 1) / QRU_CB) * batch), dim3(512),
-                               sizeof(QrUpdLds), s, buf, stride, N, W, batch, p0, nb, Vs, Ts);
+        if (reg) {
+            hipLaunchKernelGGL(qr_update_kernel, dim3(((cols + QRB - 1) / QRB) * batch), dim3(256), 0, s, buf,
+                               stride, N, W, batch, p0, nb, Vs, Wt, wt_stride, Ts);
             e = hipGetLastError();
             if (e != hipSuccess) return e;
             continue;

@@ -1,0 +1,71 @@
+"""Alternative kernel paths selected by environment switches, on the GPU.
+
+Every switch changes how a step is computed, never what it computes, so a context built under
+it must follow exactly the Markov chains of a default context with the same seeds: identical
+accept/reject decisions, and Green's functions equal to rounding (the bar 1e-9 is the wrap error
+of this workload's FP64 chains, never reached by rounding differences between two correct
+evaluation orders).  Workload: 16x16 (N = 256), U = -4, mu = -0.2, beta = 4, L = 40, pace 8,
+k = 32, 8 chains (a multiple of 8, so the chain-major block order of the batched kernels is the
+one exercised).
+
+  DQMC_QR_UPDATE=gemm    QR trailing update by three GEMM launches instead of one fused kernel
+  DQMC_WRAP=dense        slice products by dense GEMMs with exp(-dt K) instead of Kronecker MFMA
+  DQMC_TRSM=v1           column-parallel back-substitution instead of the blocked one
+  DQMC_QR=v1             one-workgroup unblocked Householder QR instead of the blocked one
+  DQMC_METRO_THREADS     Metropolis workgroup of 256 or 512 threads instead of 1024
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import dqmc_amd as gpu
+
+pytestmark = pytest.mark.gpu
+
+NL, NT, BETA, U, MU, PACE, K, NCH = 16, 40, 4.0, 4.0, -0.2, 8, 32, 8
+TOL_G = 1e-9
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_chains(n_sweeps):
+    e = gpu.Engine(NL, NT, BETA, U, mu=MU, pace=PACE, n_chains=NCH, delay_k=K)
+    for c in range(NCH):
+        e.set_seed(c, 4100 + c)
+        e.fields_random(c)
+    e.init()
+    out = []
+    for sweep in range(n_sweeps):
+        e.sweep_up() if sweep % 2 == 0 else e.sweep_down()
+        out.append(([e.accept_mask(c).copy() for c in range(NCH)], [e.gf(c).copy() for c in range(NCH)]))
+    wrap = max(e.stats(c)["wrap_eq"] for c in range(NCH))
+    e.close()
+    return out, wrap
+
+
+@pytest.fixture(scope="module")
+def default_run():
+    say("case: default kernels, 16x16 L=40, 8 chains, 2 sweeps")
+    return run_chains(2)
+
+
+@pytest.mark.parametrize("var,val", [("DQMC_QR_UPDATE", "gemm"), ("DQMC_WRAP", "dense"), ("DQMC_TRSM", "v1"),
+                                     ("DQMC_QR", "v1"), ("DQMC_METRO_THREADS", "256"),
+                                     ("DQMC_METRO_THREADS", "512")])
+def test_switch_follows_default_chains(default_run, monkeypatch, var, val):
+    say("case: %s=%s, 16x16 L=40, 8 chains, 2 sweeps" % (var, val))
+    monkeypatch.setenv(var, val)
+    alt, wrap = run_chains(2)
+    ref, ref_wrap = default_run
+    assert ref_wrap < 1e-8 and wrap < 1e-8
+    for sweep, ((ma, ga), (mr, gr)) in enumerate(zip(alt, ref)):
+        for c in range(NCH):
+            assert np.array_equal(ma[c], mr[c]), "chain %d sweep %d: accept decisions differ" % (c, sweep)
+            d = np.abs(ga[c] - gr[c]).max()
+            assert d < TOL_G, "chain %d sweep %d: max|dG| %g" % (c, sweep, d)
+        say("  sweep %d: decisions identical, max|dG| = %.2e" %
+            (sweep, max(np.abs(ga[c] - gr[c]).max() for c in range(NCH))))
