@@ -360,12 +360,124 @@ __global__ __launch_bounds__(256) void kron_right_kernel(KronArgs<T> a)
     }
 }
 
+// 16 x 16 lattice (N = 256): a lattice vector read as 64 lanes x 4 consecutive sites,
+// site = l + 64 s, is exactly the A-fragment V(x = l%16, y = 4 s + l/16) of step 1, so
+//   step 1  Y = V E^T   (A = V from a coalesced load, B(y, j) = E(j, y))
+//   step 2  Z = E Y     (A(i, x) = E(i, x), B = step 1's accumulator items)
+// and Z = E V E^T leaves the accumulator as Z(row(l, r), l%16), which a padded LDS transpose turns
+// back into consecutive sites for a coalesced store.
+template <typename T>
+__device__ __forceinline__ void kron16_vec(const T (&ef)[4], const T (&ef2)[4], const T (&v)[4],
+                                           typename Mfma<T>::acc_t& z)
+{
+    typedef typename Mfma<T>::acc_t acc_t;
+    acc_t y = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) y = Mfma<T>::mma(v[s], ef[s], y);
+    z = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) z = Mfma<T>::mma(ef2[s], y[s], z);
+}
+
+// Left product on the 16 x 16 lattice: each wave transforms 4 columns of X.
+template <typename T>
+__global__ __launch_bounds__(256) void kron_left16_kernel(KronArgs<T> a)
+{
+    __shared__ T zt[4][4][16][17];   // per wave and column: Z(i, j) at [j][i]
+    const int b = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T ef[4], ef2[4];
+    load_efrag<T>(a.E, lane, ef, ef2);
+    const T* X = a.X + (long long)b * a.sX;
+    T* C = a.C + (long long)b * a.sC;
+    const int cbase = blockIdx.x * 16 + wave * 4;
+    double din[4], dout[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int site = lane + 64 * s;
+        din[s] = (a.innd.mode != DIAG_NONE) ? diag_value(a.innd, b, site) : 1.0;
+        dout[s] = (a.rowd.mode != DIAG_NONE) ? diag_value(a.rowd, b, site) : 1.0;
+    }
+    T vin[4][4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const T v = X[(long long)(cbase + cc) * 256 + lane + 64 * s];
+            vin[cc][s] = (a.innd.mode != DIAG_NONE) ? T((double)v * din[s]) : v;
+        }
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        typename Mfma<T>::acc_t z;
+        kron16_vec<T>(ef, ef2, vin[cc], z);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) zt[wave][cc][lane & 15][Mfma<T>::row(lane, r)] = z[r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        const int c = cbase + cc;
+        const double cs = (a.cold.mode != DIAG_NONE) ? diag_value(a.cold, b, c) : 1.0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s)   // site l + 64 s = i + 16 j with i = l%16, j = l/16 + 4 s
+            C[(long long)c * 256 + lane + 64 * s] =
+                T((double)zt[wave][cc][(lane >> 4) + 4 * s][lane & 15] * cs * dout[s]);
+    }
+}
+
+// Right product on the 16 x 16 lattice: 16 rows of X per workgroup, staged row-major in LDS
+// (row stride 273, one pad word per 16 sites) so that both the lattice-vector reads and the
+// transposed accumulator writes spread over the banks; wave w transforms rows w, w+4, w+8, w+12.
+#define KR16_LD 273
+__device__ __forceinline__ int kr16_pad(int site) { return site + (site >> 4); }
+template <typename T>
+__global__ __launch_bounds__(256) void kron_right16_kernel(KronArgs<T> a)
+{
+    __shared__ T tile[16 * KR16_LD];
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = blockIdx.x * 16;
+    const T* X = a.X + (long long)b * a.sX;
+    T* C = a.C + (long long)b * a.sC;
+#pragma unroll 4
+    for (int e = tid; e < 4096; e += 256) {
+        const int rr = e & 15, c = e >> 4;
+        T v = X[r0 + rr + (long long)c * 256];
+        if (a.innd.mode != DIAG_NONE) v = T((double)v * diag_value(a.innd, b, c));
+        tile[rr * KR16_LD + kr16_pad(c)] = v;
+    }
+    __syncthreads();
+    T ef[4], ef2[4];
+    load_efrag<T>(a.E, lane, ef, ef2);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        T* row = tile + (wave + 4 * q) * KR16_LD;
+        T vin[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) vin[s] = row[kr16_pad(lane + 64 * s)];
+        typename Mfma<T>::acc_t z;
+        kron16_vec<T>(ef, ef2, vin, z);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) row[kr16_pad(Mfma<T>::row(lane, r) + 16 * (lane & 15))] = z[r];
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int e = tid; e < 4096; e += 256) {
+        const int rr = e & 15, c = e >> 4;
+        double v = (double)tile[rr * KR16_LD + kr16_pad(c)];
+        if (a.rowd.mode != DIAG_NONE) v *= diag_value(a.rowd, b, r0 + rr);
+        if (a.cold.mode != DIAG_NONE) v *= diag_value(a.cold, b, c);
+        C[r0 + rr + (long long)c * 256] = T(v);
+    }
+}
+
 template <typename T>
 hipError_t launch_kron_left(const KronArgs<T>& a, hipStream_t s)
 {
     if (a.nl > 16) return hipErrorInvalidValue;
     dim3 grid((a.N + 15) / 16, a.batch);
-    hipLaunchKernelGGL(kron_left_kernel<T>, grid, dim3(256), 0, s, a);
+    if (a.nl == 16)
+        hipLaunchKernelGGL(kron_left16_kernel<T>, grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(kron_left_kernel<T>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 template <typename T>
@@ -373,7 +485,10 @@ hipError_t launch_kron_right(const KronArgs<T>& a, hipStream_t s)
 {
     if (a.nl > 16) return hipErrorInvalidValue;
     dim3 grid((a.N + 15) / 16, a.batch);
-    hipLaunchKernelGGL(kron_right_kernel<T>, grid, dim3(256), 0, s, a);
+    if (a.nl == 16)
+        hipLaunchKernelGGL(kron_right16_kernel<T>, grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(kron_right_kernel<T>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 template hipError_t launch_kron_left<double>(const KronArgs<double>&, hipStream_t);
