@@ -424,32 +424,46 @@ __global__ __launch_bounds__(256) void kron_left16_kernel(KronArgs<T> a)
     }
 }
 
-// Right product on the 16 x 16 lattice: 16 rows of X per workgroup, staged row-major in LDS
-// (row stride 273, one pad word per 16 sites) so that both the lattice-vector reads and the
-// transposed accumulator writes spread over the banks; wave w transforms rows w, w+4, w+8, w+12.
+// Right product on the 16 x 16 lattice: 32 rows of X per workgroup (every column read and written
+// as one 256-byte segment, two rows per 16-byte access), staged row-major in LDS with row stride
+// 273 and one pad word per 16 sites so that both the lattice-vector reads and the transposed
+// accumulator writes spread over the banks; wave w of 8 transforms rows w, w+8, w+16, w+24.
 #define KR16_LD 273
+#define KR16_ROWS 32
 __device__ __forceinline__ int kr16_pad(int site) { return site + (site >> 4); }
 template <typename T>
-__global__ __launch_bounds__(256) void kron_right16_kernel(KronArgs<T> a)
+__global__ __launch_bounds__(512) void kron_right16_kernel(KronArgs<T> a)
 {
-    __shared__ T tile[16 * KR16_LD];
+    __shared__ T tile[KR16_ROWS * KR16_LD];
     const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int r0 = blockIdx.x * 16;
+    const int r0 = blockIdx.x * KR16_ROWS;
     const T* X = a.X + (long long)b * a.sX;
     T* C = a.C + (long long)b * a.sC;
 #pragma unroll 4
-    for (int e = tid; e < 4096; e += 256) {
-        const int rr = e & 15, c = e >> 4;
-        T v = X[r0 + rr + (long long)c * 256];
-        if (a.innd.mode != DIAG_NONE) v = T((double)v * diag_value(a.innd, b, c));
-        tile[rr * KR16_LD + kr16_pad(c)] = v;
+    for (int e = tid; e < KR16_ROWS / 2 * 256; e += 512) {
+        const int rr = 2 * (e & 15), c = e >> 4;
+        const T* src = X + r0 + rr + (long long)c * 256;
+        T v0, v1;
+        if (sizeof(T) == 8) {
+            const double2 v = *reinterpret_cast<const double2*>(src);
+            v0 = T(v.x); v1 = T(v.y);
+        } else {
+            const float2 v = *reinterpret_cast<const float2*>(src);
+            v0 = T(v.x); v1 = T(v.y);
+        }
+        if (a.innd.mode != DIAG_NONE) {
+            const double d = diag_value(a.innd, b, c);
+            v0 = T((double)v0 * d); v1 = T((double)v1 * d);
+        }
+        tile[rr * KR16_LD + kr16_pad(c)] = v0;
+        tile[(rr + 1) * KR16_LD + kr16_pad(c)] = v1;
     }
     __syncthreads();
     T ef[4], ef2[4];
     load_efrag<T>(a.E, lane, ef, ef2);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        T* row = tile + (wave + 4 * q) * KR16_LD;
+    for (int q = 0; q < KR16_ROWS / 8; ++q) {
+        T* row = tile + (wave + 8 * q) * KR16_LD;
         T vin[4];
 #pragma unroll
         for (int s = 0; s < 4; ++s) vin[s] = row[kr16_pad(lane + 64 * s)];
@@ -460,12 +474,25 @@ __global__ __launch_bounds__(256) void kron_right16_kernel(KronArgs<T> a)
     }
     __syncthreads();
 #pragma unroll 4
-    for (int e = tid; e < 4096; e += 256) {
-        const int rr = e & 15, c = e >> 4;
-        double v = (double)tile[rr * KR16_LD + kr16_pad(c)];
-        if (a.rowd.mode != DIAG_NONE) v *= diag_value(a.rowd, b, r0 + rr);
-        if (a.cold.mode != DIAG_NONE) v *= diag_value(a.cold, b, c);
-        C[r0 + rr + (long long)c * 256] = T(v);
+    for (int e = tid; e < KR16_ROWS / 2 * 256; e += 512) {
+        const int rr = 2 * (e & 15), c = e >> 4;
+        double v0 = (double)tile[rr * KR16_LD + kr16_pad(c)], v1 = (double)tile[(rr + 1) * KR16_LD + kr16_pad(c)];
+        if (a.rowd.mode != DIAG_NONE) {
+            v0 *= diag_value(a.rowd, b, r0 + rr);
+            v1 *= diag_value(a.rowd, b, r0 + rr + 1);
+        }
+        if (a.cold.mode != DIAG_NONE) {
+            const double d = diag_value(a.cold, b, c);
+            v0 *= d; v1 *= d;
+        }
+        T* dst = C + r0 + rr + (long long)c * 256;
+        if (sizeof(T) == 8) {
+            double2 v; v.x = v0; v.y = v1;
+            *reinterpret_cast<double2*>(dst) = v;
+        } else {
+            float2 v; v.x = (float)v0; v.y = (float)v1;
+            *reinterpret_cast<float2*>(dst) = v;
+        }
     }
 }
 
@@ -486,7 +513,7 @@ hipError_t launch_kron_right(const KronArgs<T>& a, hipStream_t s)
     if (a.nl > 16) return hipErrorInvalidValue;
     dim3 grid((a.N + 15) / 16, a.batch);
     if (a.nl == 16)
-        hipLaunchKernelGGL(kron_right16_kernel<T>, grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL(kron_right16_kernel<T>, dim3(256 / KR16_ROWS, a.batch), dim3(512), 0, s, a);
     else
         hipLaunchKernelGGL(kron_right_kernel<T>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
