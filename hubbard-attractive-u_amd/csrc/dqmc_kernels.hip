@@ -843,6 +843,53 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Register-only cross-lane moves of f64 values (DPP and the gfx950 permlane swaps), measured lane
+// patterns: quad_perm [1,0,3,2] = lane ^ 1, quad_perm [2,3,0,1] = lane ^ 2, row_half_mirror = lane ^ 7
+// within 8 lanes, row_mirror = lane ^ 15 within 16 lanes; permlane16_swap(v, v) returns the values
+// of the even and the odd row of each row pair, permlane32_swap(v, v) those of the two halves.
+#define DPP_XOR1 0xB1
+#define DPP_XOR2 0x4E
+#define DPP_HALF_MIRROR 0x141
+#define DPP_MIRROR 0x140
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// v(lane) + v(lane ^ 16), added in the same order in both lanes
+__device__ __forceinline__ double sum_xor16(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(u >> 32), (unsigned)(u >> 32), false, false);
+    const double even = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    const double odd = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+    return even + odd;
+}
+// v(lane) + v(lane ^ 32), added in the same order in both lanes
+__device__ __forceinline__ double sum_xor32(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)u, (unsigned)u, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(u >> 32), (unsigned)(u >> 32), false, false);
+    const double low = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    const double high = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+    return low + high;
+}
+// sum over the 16 lanes of a row, bit-identical in all of them
+__device__ __forceinline__ double row_sum_dpp(double v)
+{
+    v += dpp_f64<DPP_XOR1>(v);
+    v += dpp_f64<DPP_XOR2>(v);
+    v += dpp_f64<DPP_HALF_MIRROR>(v);
+    v += dpp_f64<DPP_MIRROR>(v);
+    return v;
+}
+__device__ __forceinline__ double wave_sum_dpp(double v) { return sum_xor32(sum_xor16(row_sum_dpp(v))); }
+
 // Blocked Householder QR.  Per panel of QR_NB columns:
 //   1. the panel (rows p0..N-1) is factored in LDS, one reflector per column (dlarfg);
 //   2. its R rows go back to global memory, the reflectors V (unit lower trapezoid, explicit
@@ -1253,7 +1300,7 @@ __global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bu
         const bool owner = g == go;
         if (owner) xcol[r] = live ? xi : 0.0;
         double n2 = (owner && live && r > i) ? xi * xi : 0.0;
-        n2 = wave_sum(n2);
+        n2 = wave_sum_dpp(n2);
         if (lane == 0) red1[wave] = n2;
         __syncthreads();
         double s2 = 0.0;
@@ -1284,29 +1331,30 @@ __global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bu
             else if (col > i && col < nb) vm = x[m];
             d[m] = vm * vr;
         }
-        // recursive halving over the 64 lanes: 8 values -> lane owns index (lane >> 3) & 7
+        // recursive halving over the 64 lanes, register-only: 8 values -> 4 (pairs across lane bit 3),
+        // 2 (bit 2), 1 (bit 1), then plain sums across bits 0, 4 and 5; lane owns index
+        // 4 bit3 + 2 bit2 + bit1.
         {
             double w4[4], w2[2], w1;
-            const bool h32 = lane & 32, h16 = lane & 16, h8 = lane & 8;
+            const bool h8 = lane & 8, h4 = lane & 4, h2 = lane & 2;
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-                const double keep = h32 ? d[m + 4] : d[m], send = h32 ? d[m] : d[m + 4];
-                w4[m] = keep + __shfl_xor(send, 32, 64);
+                const double keep = h8 ? d[m + 4] : d[m], send = h8 ? d[m] : d[m + 4];
+                w4[m] = keep + dpp_f64<DPP_MIRROR>(send);
             }
 #pragma unroll
             for (int m = 0; m < 2; ++m) {
-                const double keep = h16 ? w4[m + 2] : w4[m], send = h16 ? w4[m] : w4[m + 2];
-                w2[m] = keep + __shfl_xor(send, 16, 64);
+                const double keep = h4 ? w4[m + 2] : w4[m], send = h4 ? w4[m] : w4[m + 2];
+                w2[m] = keep + dpp_f64<DPP_HALF_MIRROR>(send);
             }
             {
-                const double keep = h8 ? w2[1] : w2[0], send = h8 ? w2[0] : w2[1];
-                w1 = keep + __shfl_xor(send, 8, 64);
+                const double keep = h2 ? w2[1] : w2[0], send = h2 ? w2[0] : w2[1];
+                w1 = keep + dpp_f64<DPP_XOR2>(send);
             }
-            w1 += __shfl_xor(w1, 4, 64);
-            w1 += __shfl_xor(w1, 2, 64);
-            w1 += __shfl_xor(w1, 1, 64);
-            const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
-            if ((lane & 7) == 0) red[wave][idx] = w1;
+            w1 += dpp_f64<DPP_XOR1>(w1);
+            w1 = sum_xor32(sum_xor16(w1));
+            const int idx = (h8 ? 4 : 0) + (h4 ? 2 : 0) + (h2 ? 1 : 0);
+            if ((lane & 0x31) == 0) red[wave][idx] = w1;
         }
         __syncthreads();
 #pragma unroll
@@ -1342,26 +1390,18 @@ __global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bu
             }
         }
     }
-    // T row j by thread j: T(j, j) = tau_j, T(j, i) = -tau_i sum_{k=j}^{i-1} T(j, k) VtV(k, i)
-    if (t < QRB) {
-        const int j = t;
-        double trow[QRB];
-#pragma unroll
-        for (int i = 0; i < QRB; ++i) {
-            double v = 0.0;
-            if (i == j && i < nb) {
-                v = tau_s[i];
-            } else if (i > j && i < nb) {
-                double acc = 0.0;
-#pragma unroll
-                for (int k = 0; k < QRB; ++k)
-                    if (k >= j && k < i) acc += trow[k] * VtV[k][i];
-                v = -tau_s[i] * acc;
-            }
-            trow[i] = v;
+    // T(j, j) = tau_j, T(j, i) = -tau_i sum_{k=j}^{i-1} T(j, k) VtV(k, i): lane (j, k) = (t / 32, t % 32)
+    // holds T(j, k); the 32 column steps each sum one row over its 32 lanes in registers.
+    {
+        const int j = t >> 5, k = t & 31;
+        double tk = 0.0;
+        for (int i = 0; i < nb; ++i) {
+            double c = (k >= j && k < i) ? tk * VtV[k][i] : 0.0;
+            c = sum_xor16(row_sum_dpp(c));
+            const double tji = (i == j) ? tau_s[i] : ((i > j) ? -tau_s[i] * c : 0.0);
+            if (k == i) tk = tji;
         }
-#pragma unroll
-        for (int i = 0; i < QRB; ++i) Tg[j + (long long)i * QRB] = trow[i];
+        Tg[j + (long long)k * QRB] = tk;
     }
 }
 
