@@ -565,13 +565,15 @@ template <typename T>
 struct MetroLds {
     T* Xl;         // [kb][N]  X^T of the pending rank-a update (rows m, G-row index contiguous)
     T* Yl;         // [kb][N]  Y (rows m, G-column index contiguous)
-    double* Gs;    // [kb][kb+1] current G restricted to the site block
+    double* colq;  // [32] column q of the current G(S,S), broadcast inside the site-loop wave
+    double* rowq;  // [32] row q of the current G(S,S)
     double* XS;    // [kb][kb]  gamma_m * G(S, i_m) of accepted flip m (restricted to the block)
     double* YS;    // [kb][kb]  G(i_m, S) - e_{i_m} (restricted to the block)
     double* unif;  // [N]
     double* gam;   // [kb]
     int* sl;       // [N] fields of the slice
     int* ql;       // [kb] block-local index of accepted flip m
+    int* nacc;     // [1] accepted flips of the current block
     uint32_t* mask;
 };
 
@@ -582,14 +584,16 @@ __device__ MetroLds<T> metro_lds(unsigned char* smem, int N, int kb)
     const int kb4 = ((kb + 3) / 4) * 4;
     L.Xl = reinterpret_cast<T*>(smem);
     L.Yl = L.Xl + (size_t)kb4 * N;
-    L.Gs = reinterpret_cast<double*>(L.Yl + (size_t)kb4 * N);
-    L.XS = L.Gs + (size_t)kb * (kb + 1);
+    L.colq = reinterpret_cast<double*>(L.Yl + (size_t)kb4 * N);
+    L.rowq = L.colq + 32;
+    L.XS = L.rowq + 32;
     L.YS = L.XS + (size_t)kb * kb;
     L.unif = L.YS + (size_t)kb * kb;
     L.gam = L.unif + N;
     L.sl = reinterpret_cast<int*>(L.gam + kb);
     L.ql = L.sl + N;
-    L.mask = reinterpret_cast<uint32_t*>(L.ql + kb);
+    L.nacc = L.ql + kb;
+    L.mask = reinterpret_cast<uint32_t*>(L.nacc + 1);
     return L;
 }
 
@@ -598,8 +602,8 @@ size_t metropolis_lds_bytes(int N, int kb, size_t esz)
     const int kb4 = ((kb + 3) / 4) * 4;
     size_t b = (size_t)2 * kb4 * N * esz;
     b = (b + 7) / 8 * 8;
-    b += (size_t)8 * ((size_t)kb * (kb + 1) + 2 * (size_t)kb * kb + N + kb);
-    b += (size_t)4 * (N + kb) + (size_t)4 * ((N + 31) / 32) + 64;
+    b += (size_t)8 * (64 + 2 * (size_t)kb * kb + N + kb);
+    b += (size_t)4 * (N + kb + 1) + (size_t)4 * ((N + 31) / 32) + 64;
     return b;
 }
 
@@ -653,6 +657,94 @@ __device__ __forceinline__ void flush_mfma(T* __restrict__ G, int N, const T* __
     }
 }
 
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// LDS writes of some lanes of a wave become visible to the later LDS reads of the other lanes
+// of the same wave (no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Accept/reject sequence of one site block S = b0 .. b0+bs-1 (bs <= 32) by one wave, with the
+// reference's rank-1 update applied to G(S,S) after every accepted flip.  Lane l holds row
+// r = l % 32 and columns 16 (l / 32) .. 16 (l / 32) + 15 of G(S,S) in registers, plus its own
+// diagonal entry G(r, r) (kept by the same operations, so bit-identical to the block entry).  An
+// accepted flip q publishes column q and row q through LDS; every lane then applies
+// G(r, c) += (gamma G(r, q)) (G(q, c) - delta_qc).  Records XS, YS, gam, ql and the flip count for
+// the rebuild, and flips the fields.
+template <typename T>
+__device__ __forceinline__ void metro_site_block(const MetroLds<T>& L, const T* __restrict__ G, int8_t* f, int N,
+                                                 int kb, int b0, int bs, double dpos, double dneg)
+{
+    const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
+    double g[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int c = 16 * h + j;
+        g[j] = (r < bs && c < bs) ? (double)G[(b0 + r) + (long long)(b0 + c) * N] : 0.0;
+    }
+    double dg = (r < bs) ? (double)G[(b0 + r) + (long long)(b0 + r) * N] : 0.0;
+    const double uq = (lane < bs) ? L.unif[b0 + lane] : 0.0;
+    const int sq = (lane < bs) ? L.sl[b0 + lane] : 0;
+    int na = 0;
+    for (int qh = 0; qh < 2; ++qh) {
+        if (16 * qh >= bs) break;
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const int q = 16 * qh + jj;
+            if (q >= bs) continue;
+            const double gii = readlane_f64(dg, q);
+            const double u = readlane_f64(uq, q);
+            const int si = __builtin_amdgcn_readlane(sq, q);
+            const double delta = si > 0 ? dpos : dneg;
+            double x;
+            const double p = accept_prob(gii, delta, &x);
+            if (u < p) {
+                const double gam = delta / x;
+                if (h == qh) L.colq[r] = g[jj];
+                if (r == q) {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) L.rowq[16 * h + j] = g[j];
+                }
+                wave_lds_sync();
+                const double xs = gam * L.colq[r];
+                const double ysd = L.rowq[r] - (r == q ? 1.0 : 0.0);
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int c = 16 * h + j;
+                    const double ys = L.rowq[c] - (c == q ? 1.0 : 0.0);
+                    g[j] += xs * ys;
+                }
+                dg += xs * ysd;
+                if (lane < bs) {
+                    L.XS[na * kb + lane] = xs;
+                    L.YS[na * kb + lane] = ysd;
+                }
+                if (lane == 0) {
+                    const int i = b0 + q;
+                    L.gam[na] = gam;
+                    L.ql[na] = q;
+                    L.sl[i] = -si;
+                    f[i] = (int8_t)(-si);
+                    L.mask[i >> 5] |= 1u << (i & 31);
+                }
+                ++na;
+                wave_lds_sync();
+            }
+        }
+    }
+    if (lane == 0) *L.nacc = na;
+}
+
 // Submatrix Metropolis update of slice t.  Sites are visited in blocks S of kb consecutive
 // sites.  Inside a block the accept/reject sequence runs on Gs = G(S,S) in LDS with the
 // reference's rank-1 update.  After the block the accepted flips A are applied to the whole
@@ -688,44 +780,9 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
     for (int b0 = 0; b0 < N; b0 += kb) {
         const int bs = min(kb, N - b0);
         __syncthreads();
-        for (int e = tid; e < bs * bs; e += nth) {
-            const int r = e % bs, c = e / bs;
-            L.Gs[r * (kb + 1) + c] = (double)G[(b0 + r) + (long long)(b0 + c) * N];
-        }
+        if (tid < 64) metro_site_block(L, G, f, N, kb, b0, bs, dpos, dneg);
         __syncthreads();
-        int na = 0;
-        for (int q = 0; q < bs; ++q) {
-            const int i = b0 + q;
-            const double gii = L.Gs[q * (kb + 1) + q];
-            const int si = L.sl[i];
-            const double delta = si > 0 ? dpos : dneg;
-            double x;
-            const double p = accept_prob(gii, delta, &x);
-            if (L.unif[i] < p) {
-                const double gam = delta / x;
-                if (tid < bs) {
-                    L.XS[na * kb + tid] = gam * L.Gs[tid * (kb + 1) + q];
-                    L.YS[na * kb + tid] = L.Gs[q * (kb + 1) + tid] - (tid == q ? 1.0 : 0.0);
-                }
-                if (tid == 0) {
-                    L.gam[na] = gam;
-                    L.ql[na] = q;
-                }
-                // every wave has read sl[i] for this site once all are past this barrier
-                __syncthreads();
-                if (tid == 0) {
-                    L.sl[i] = -si;
-                    f[i] = (int8_t)(-si);
-                    L.mask[i >> 5] |= 1u << (i & 31);
-                }
-                for (int e = tid; e < bs * bs; e += nth) {
-                    const int r = e % bs, c = e / bs;
-                    L.Gs[r * (kb + 1) + c] += L.XS[na * kb + r] * L.YS[na * kb + c];
-                }
-                ++na;
-                __syncthreads();
-            }
-        }
+        const int na = *L.nacc;
         nacc += na;
         if (na == 0) continue;
         // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  The two recursions
