@@ -677,10 +677,12 @@ __device__ __forceinline__ void wave_lds_sync()
 // Accept/reject sequence of one site block S = b0 .. b0+bs-1 (bs <= 32) by one wave, with the
 // reference's rank-1 update applied to G(S,S) after every accepted flip.  Lane l holds row
 // r = l % 32 and columns 16 (l / 32) .. 16 (l / 32) + 15 of G(S,S) in registers, plus its own
-// diagonal entry G(r, r) (kept by the same operations, so bit-identical to the block entry).  An
-// accepted flip q publishes column q and row q through LDS; every lane then applies
-// G(r, c) += (gamma G(r, q)) (G(q, c) - delta_qc).  Records XS, YS, gam, ql and the flip count for
-// the rebuild, and flips the fields.
+// diagonal entry G(r, r) (kept by the same operations, so bit-identical to the block entry), and
+// the acceptance probability and gamma of site r computed from it.  They change only when a flip
+// is accepted, so every lane recomputes them then (in parallel), and a rejected site costs two
+// lane reads and a compare.  An accepted flip q publishes column q and row q through LDS; every
+// lane then applies G(r, c) += (gamma G(r, q)) (G(q, c) - delta_qc).  Records XS, YS, gam, ql and
+// the flip count for the rebuild, and flips the fields.
 template <typename T>
 __device__ __forceinline__ void metro_site_block(const MetroLds<T>& L, const T* __restrict__ G, int8_t* f, int N,
                                                  int kb, int b0, int bs, double dpos, double dneg)
@@ -693,8 +695,10 @@ __device__ __forceinline__ void metro_site_block(const MetroLds<T>& L, const T* 
         g[j] = (r < bs && c < bs) ? (double)G[(b0 + r) + (long long)(b0 + c) * N] : 0.0;
     }
     double dg = (r < bs) ? (double)G[(b0 + r) + (long long)(b0 + r) * N] : 0.0;
-    const double uq = (lane < bs) ? L.unif[b0 + lane] : 0.0;
-    const int sq = (lane < bs) ? L.sl[b0 + lane] : 0;
+    const double uq = (r < bs) ? L.unif[b0 + r] : 0.0;
+    const int sr = (r < bs) ? L.sl[b0 + r] : 1;
+    const double dr = sr > 0 ? dpos : dneg;
+    double xr, pr = accept_prob(dg, dr, &xr), gr = dr / xr;
     int na = 0;
     for (int qh = 0; qh < 2; ++qh) {
         if (16 * qh >= bs) break;
@@ -702,14 +706,8 @@ __device__ __forceinline__ void metro_site_block(const MetroLds<T>& L, const T* 
         for (int jj = 0; jj < 16; ++jj) {
             const int q = 16 * qh + jj;
             if (q >= bs) continue;
-            const double gii = readlane_f64(dg, q);
-            const double u = readlane_f64(uq, q);
-            const int si = __builtin_amdgcn_readlane(sq, q);
-            const double delta = si > 0 ? dpos : dneg;
-            double x;
-            const double p = accept_prob(gii, delta, &x);
-            if (u < p) {
-                const double gam = delta / x;
+            if (readlane_f64(uq, q) < readlane_f64(pr, q)) {
+                const double gam = readlane_f64(gr, q);
                 if (h == qh) L.colq[r] = g[jj];
                 if (r == q) {
 #pragma unroll
@@ -725,12 +723,15 @@ __device__ __forceinline__ void metro_site_block(const MetroLds<T>& L, const T* 
                     g[j] += xs * ys;
                 }
                 dg += xs * ysd;
+                pr = accept_prob(dg, dr, &xr);
+                gr = dr / xr;
                 if (lane < bs) {
                     L.XS[na * kb + lane] = xs;
                     L.YS[na * kb + lane] = ysd;
                 }
                 if (lane == 0) {
                     const int i = b0 + q;
+                    const int si = L.sl[i];
                     L.gam[na] = gam;
                     L.ql[na] = q;
                     L.sl[i] = -si;
