@@ -607,51 +607,92 @@ size_t metropolis_lds_bytes(int N, int kb, size_t esz)
     return b;
 }
 
-// G(ri, cj) += sum_m X[ri][m] Y[m][cj] over the whole N x N matrix, on MFMA.  The tile is
-// computed transposed (MFMA rows = G columns) so that each lane group touches contiguous
-// column segments of the column-major G.  Each wave keeps FT tiles in flight (all loads of a
-// group are issued before its first MFMA) so that the global-memory latency overlaps; FT is 8,
-// or 4 under the 128-register budget of a 1024-thread workgroup.
+// Tile sets of the flush, in 16 x 16 tiles (tile column ct, tile row rt; nt per side).  The cross of
+// tile rows/columns tr0..tr1 holds the rows and columns of the next site block, which its site loop
+// and rebuild read; the rest is everything else.
+enum FlushTiles : int { FLUSH_ALL = 0, FLUSH_CROSS = 1, FLUSH_REST = 2 };
+__device__ __forceinline__ int flush_tile_count(int mode, int nt, int ncr)
+{
+    if (mode == FLUSH_ALL) return nt * nt;
+    if (mode == FLUSH_CROSS) return ncr * nt + ncr * (nt - ncr);
+    return (nt - ncr) * (nt - ncr);
+}
+__device__ __forceinline__ void flush_tile(int mode, int u, int nt, int tr0, int ncr, int& ct, int& rt)
+{
+    if (mode == FLUSH_ALL) {
+        ct = u / nt;
+        rt = u % nt;
+    } else if (mode == FLUSH_CROSS) {
+        if (u < ncr * nt) {
+            ct = tr0 + u / nt;
+            rt = u % nt;
+        } else {
+            const int v = u - ncr * nt, o = nt - ncr;
+            rt = tr0 + v / o;
+            ct = v % o;
+            if (ct >= tr0) ct += ncr;
+        }
+    } else {
+        const int o = nt - ncr;
+        ct = u / o;
+        rt = u % o;
+        if (ct >= tr0) ct += ncr;
+        if (rt >= tr0) rt += ncr;
+    }
+}
+
+// G(ri, cj) += sum_m X[ri][m] Y[m][cj] over one tile set of the N x N matrix, on MFMA, by waves
+// w0 .. of the workgroup.  The tile is computed transposed (MFMA rows = G columns) so that each
+// lane group touches contiguous column segments of the column-major G.  Each wave keeps FT tiles
+// in flight (all loads of a group are issued before its first MFMA) so that the global-memory
+// latency overlaps; FT is 8, or 4 under the 128-register budget of a 1024-thread workgroup.
 template <typename T, int FT>
-__device__ __forceinline__ void flush_mfma(T* __restrict__ G, int N, const T* __restrict__ Xl, const T* __restrict__ Yl, int a4)
+__device__ __forceinline__ void flush_mfma(T* __restrict__ G, int N, const T* __restrict__ Xl, const T* __restrict__ Yl,
+                                           int a4, int mode, int tr0, int tr1, int w0)
 {
     typedef typename Mfma<T>::acc_t acc_t;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int nt = (N + 15) / 16, ntt = nt * nt;
+    const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) - w0, nw = (blockDim.x >> 6) - w0;
+    const int nt = (N + 15) / 16, ncr = (mode == FLUSH_ALL) ? 0 : tr1 - tr0 + 1;
+    const int ntt = flush_tile_count(mode, nt, ncr);
     for (int t0 = wave * FT; t0 < ntt; t0 += nw * FT) {
         acc_t acc[FT];
+        int c0s[FT], r0s[FT];
 #pragma unroll
         for (int u = 0; u < FT; ++u) {
-            const int tile = t0 + u;
-            const int c0 = (tile / nt) * 16, r0 = (tile % nt) * 16;
+            int ct = 0, rt = 0;
+            if (t0 + u < ntt) flush_tile(mode, t0 + u, nt, tr0, ncr, ct, rt);
+            c0s[u] = (t0 + u < ntt) ? ct * 16 : N;   // N marks an empty slot
+            r0s[u] = rt * 16;
+        }
+#pragma unroll
+        for (int u = 0; u < FT; ++u) {
+            const int c0 = c0s[u], r0 = r0s[u];
             const int gr = r0 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gc = c0 + Mfma<T>::row(lane, r);
-                acc[u][r] = (tile < ntt && gr < N && gc < N) ? G[gr + (long long)gc * N] : T(0);
+                acc[u][r] = (gr < N && gc < N) ? G[gr + (long long)gc * N] : T(0);
             }
         }
         for (int k0 = 0; k0 < a4; k0 += 4) {
             const int kk = k0 + (lane >> 4);
 #pragma unroll
             for (int u = 0; u < FT; ++u) {
-                const int tile = t0 + u;
-                const int c0 = (tile / nt) * 16, r0 = (tile % nt) * 16;
+                const int c0 = c0s[u], r0 = r0s[u];
                 const int ca = c0 + (lane & 15), gr = r0 + (lane & 15);
-                const T av = (tile < ntt && ca < N) ? Yl[kk * N + ca] : T(0);   // A(row = G column, k)
-                const T bv = (tile < ntt && gr < N) ? Xl[kk * N + gr] : T(0);   // B(k, col = G row)
+                const T av = (ca < N) ? Yl[kk * N + ca] : T(0);   // A(row = G column, k)
+                const T bv = (c0 < N && gr < N) ? Xl[kk * N + gr] : T(0);   // B(k, col = G row)
                 acc[u] = Mfma<T>::mma(av, bv, acc[u]);
             }
         }
 #pragma unroll
         for (int u = 0; u < FT; ++u) {
-            const int tile = t0 + u;
-            const int c0 = (tile / nt) * 16, r0 = (tile % nt) * 16;
+            const int c0 = c0s[u], r0 = r0s[u];
             const int gr = r0 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int gc = c0 + Mfma<T>::row(lane, r);
-                if (tile < ntt && gr < N && gc < N) G[gr + (long long)gc * N] = acc[u][r];
+                if (gr < N && gc < N) G[gr + (long long)gc * N] = acc[u][r];
             }
         }
     }
@@ -778,66 +819,84 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
 
     const double dpos = a.delta[2 * b], dneg = a.delta[2 * b + 1];
     int nacc = 0;
+    // Software pipeline over the site blocks: after block B's rebuild, the flush first updates the
+    // cross of rows and columns of block B+1 (all waves); then wave 0 runs block B+1's site loop,
+    // which reads only that cross, while the other waves flush the rest of G.
+    __syncthreads();
+    if (tid < 64) metro_site_block(L, G, f, N, kb, 0, min(kb, N), dpos, dneg);
+    __syncthreads();
     for (int b0 = 0; b0 < N; b0 += kb) {
         const int bs = min(kb, N - b0);
-        __syncthreads();
-        if (tid < 64) metro_site_block(L, G, f, N, kb, b0, bs, dpos, dneg);
-        __syncthreads();
         const int na = *L.nacc;
         nacc += na;
-        if (na == 0) continue;
-        // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  The two recursions
-        // are independent: with at least 2N threads one half builds X and the other half Y,
-        // each in registers with the recursion unrolled.  The Y threads read the block's rows
-        // G0(S, j) as one contiguous segment (staged in their own column j of Yl) instead of na
-        // scattered row reads.
-        const int a4 = ((na + 3) / 4) * 4;
-        const bool split = nth >= 2 * N;
-        for (int e = tid; e < (split ? 2 * N : N); e += nth) {
-            const int j = split ? (e % N) : e;
-            const bool do_x = !split || e < N, do_y = !split || e >= N;
-            if (do_x) {
-                double xr[MB_MAX];
+        const int b1 = b0 + kb, bs1 = (b1 < N) ? min(kb, N - b1) : 0;
+        const bool next = b1 < N;
+        const int tr0 = next ? b1 / 16 : 0, tr1 = next ? (b1 + bs1 - 1) / 16 : 0;
+        if (na > 0) {
+            // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  The two recursions
+            // are independent: with at least 2N threads one half builds X and the other half Y,
+            // each in registers with the recursion unrolled.  The Y threads read the block's rows
+            // G0(S, j) as one contiguous segment (staged in their own column j of Yl) instead of na
+            // scattered row reads.
+            const int a4 = ((na + 3) / 4) * 4;
+            const bool split = nth >= 2 * N;
+            for (int e = tid; e < (split ? 2 * N : N); e += nth) {
+                const int j = split ? (e % N) : e;
+                const bool do_x = !split || e < N, do_y = !split || e >= N;
+                if (do_x) {
+                    double xr[MB_MAX];
 #pragma unroll
-                for (int m = 0; m < MB_MAX; ++m)
-                    xr[m] = (m < na) ? (double)G[j + (long long)(b0 + L.ql[m]) * N] : 0.0;
+                    for (int m = 0; m < MB_MAX; ++m)
+                        xr[m] = (m < na) ? (double)G[j + (long long)(b0 + L.ql[m]) * N] : 0.0;
 #pragma unroll
-                for (int m = 0; m < MB_MAX; ++m) {
-                    if (m < na) {
-                        const int qm = L.ql[m];
-                        double sx = xr[m];
+                    for (int m = 0; m < MB_MAX; ++m) {
+                        if (m < na) {
+                            const int qm = L.ql[m];
+                            double sx = xr[m];
 #pragma unroll
-                        for (int mp = 0; mp < m; ++mp) sx += xr[mp] * L.YS[mp * kb + qm];
-                        xr[m] = L.gam[m] * sx;
+                            for (int mp = 0; mp < m; ++mp) sx += xr[mp] * L.YS[mp * kb + qm];
+                            xr[m] = L.gam[m] * sx;
+                        }
                     }
-                }
 #pragma unroll
-                for (int m = 0; m < MB_MAX; ++m)
-                    if (m < a4) L.Xl[m * N + j] = T(xr[m]);
+                    for (int m = 0; m < MB_MAX; ++m)
+                        if (m < a4) L.Xl[m * N + j] = T(xr[m]);
+                }
+                if (do_y) {
+                    for (int c = 0; c < bs; ++c) L.Yl[c * N + j] = G[(b0 + c) + (long long)j * N];
+                    double yr[MB_MAX];
+#pragma unroll
+                    for (int m = 0; m < MB_MAX; ++m)
+                        yr[m] = (m < na) ? (double)L.Yl[L.ql[m] * N + j] - (j == b0 + L.ql[m] ? 1.0 : 0.0) : 0.0;
+#pragma unroll
+                    for (int m = 0; m < MB_MAX; ++m) {
+                        if (m < na) {
+                            const int qm = L.ql[m];
+                            double sy = yr[m];
+#pragma unroll
+                            for (int mp = 0; mp < m; ++mp) sy += L.XS[mp * kb + qm] * yr[mp];
+                            yr[m] = sy;
+                        }
+                    }
+#pragma unroll
+                    for (int m = 0; m < MB_MAX; ++m)
+                        if (m < a4) L.Yl[m * N + j] = T(yr[m]);
+                }
             }
-            if (do_y) {
-                for (int c = 0; c < bs; ++c) L.Yl[c * N + j] = G[(b0 + c) + (long long)j * N];
-                double yr[MB_MAX];
-#pragma unroll
-                for (int m = 0; m < MB_MAX; ++m)
-                    yr[m] = (m < na) ? (double)L.Yl[L.ql[m] * N + j] - (j == b0 + L.ql[m] ? 1.0 : 0.0) : 0.0;
-#pragma unroll
-                for (int m = 0; m < MB_MAX; ++m) {
-                    if (m < na) {
-                        const int qm = L.ql[m];
-                        double sy = yr[m];
-#pragma unroll
-                        for (int mp = 0; mp < m; ++mp) sy += L.XS[mp * kb + qm] * yr[mp];
-                        yr[m] = sy;
-                    }
-                }
-#pragma unroll
-                for (int m = 0; m < MB_MAX; ++m)
-                    if (m < a4) L.Yl[m * N + j] = T(yr[m]);
+            __syncthreads();
+            if (next) {
+                flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, a4, FLUSH_CROSS, tr0, tr1, 0);
+                __syncthreads();
             }
         }
-        __syncthreads();
-        flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, a4);
+        if (next) {
+            if (tid < 64)
+                metro_site_block(L, G, f, N, kb, b1, bs1, dpos, dneg);
+            else if (na > 0)
+                flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, ((na + 3) / 4) * 4, FLUSH_REST, tr0, tr1, 1);
+        } else if (na > 0) {
+            flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, ((na + 3) / 4) * 4, FLUSH_ALL, 0, 0, 0);
+        }
         __syncthreads();
     }
     __syncthreads();
