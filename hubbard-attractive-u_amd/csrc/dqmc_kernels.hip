@@ -1530,11 +1530,11 @@ __global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bu
 // That is also the A-fragment of At^T for the k-step rows 16 rt + 4 r .. + 3, so
 //   1. each wave forms its partial W = At^T V (32 x 32) over its own rows, V read row-major;
 //   2. the four partials are summed in LDS in a fixed order (the only LDS traffic);
-//   3. every wave forms W2^T = T^T W^T, whose accumulator items are the B-fragments of step 4;
-//   4. At -= V W2^T on its own tiles (V read column-major: 16 consecutive rows per fragment),
-//      and stores them.
+//   3. wave w forms tile (w / 2, w % 2) of W2^T = T^T W^T into LDS;
+//   4. At -= V W2^T on its own tiles (V read column-major: 16 consecutive rows per fragment, the
+//      B-fragments of W2^T from LDS), and stores them.
 // At is read and written exactly once; V and T come from L2.
-__global__ __launch_bounds__(256) void qr_update_kernel(double* __restrict__ bufs, long long stride, int N,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void qr_update_kernel(double* __restrict__ bufs, long long stride, int N,
                                                             int W, int batch, int p0, int nb,
                                                             const double* __restrict__ Vs,
                                                             const double* __restrict__ Vrows, long long vr_stride,
@@ -1602,22 +1602,22 @@ __global__ __launch_bounds__(256) void qr_update_kernel(double* __restrict__ buf
         Wp[0][i][m] = ((Wp[0][i][m] + Wp[1][i][m]) + Wp[2][i][m]) + Wp[3][i][m];
     }
     __syncthreads();
-    // 3. W2^T(m, i) = sum_j T(j, m) W(i, j); item r of w2t[mt][it] is W2^T(16 mt + 4 r + l/16, 16 it + l%16)
-    acc_t w2t[2][2];
+    // 3. W2^T(m, i) = sum_j T(j, m) W(i, j), tile (mt, it) = (wave / 2, wave % 2) by wave; stored
+    //    negated in Wp[1] as W2n[m][i]
+    {
+        const int mt = wave >> 1, it = wave & 1;
+        acc_t acc = acc_t{0, 0, 0, 0};
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-        for (int it = 0; it < 2; ++it) {
-            acc_t acc = acc_t{0, 0, 0, 0};
-#pragma unroll
-            for (int j0 = 0; j0 < QRB; j0 += 4) {
-                const int j = j0 + lr, m = 16 * mt + lc;
-                const double av = (j < nb && m < nb) ? T[j + (long long)m * QRB] : 0.0;
-                const double bv = Wp[0][16 * it + lc][j];
-                acc = Mfma<double>::mma(av, bv, acc);
-            }
-            w2t[mt][it] = -acc;
+        for (int j0 = 0; j0 < QRB; j0 += 4) {
+            const int j = j0 + lr, m = 16 * mt + lc;
+            const double av = (j < nb && m < nb) ? T[j + (long long)m * QRB] : 0.0;
+            const double bv = Wp[0][16 * it + lc][j];
+            acc = Mfma<double>::mma(av, bv, acc);
         }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Wp[1][16 * mt + 4 * r + lr][16 * it + lc] = -acc[r];
+    }
+    __syncthreads();
     // 4. At -= V W2^T on this wave's tiles
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -1637,7 +1637,8 @@ __global__ __launch_bounds__(256) void qr_update_kernel(double* __restrict__ buf
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc = Mfma<double>::mma(av[mt][r], w2t[mt][ct][r], acc);
+                for (int r = 0; r < 4; ++r)   // B(k = m, n = i) = -W2^T(16 mt + 4 r + l/16, 16 ct + l%16)
+                    acc = Mfma<double>::mma(av[mt][r], Wp[1][16 * mt + 4 * r + lr][16 * ct + lc], acc);
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * rt + 4 * r + lr, col = 16 * ct + lc;
