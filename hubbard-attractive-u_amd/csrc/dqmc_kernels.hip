@@ -73,6 +73,23 @@ __device__ __forceinline__ double diag_value(const DiagArg& d, int b, int i)
     }
 }
 
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// LDS writes of some lanes of a wave become visible to the later LDS reads of the other lanes
+// of the same wave (no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---------------------------------------------------------------------------------------------
 //  GEMM.  256 threads = 4 waves in a 2 x 2 grid; each wave owns a (BM/2) x (BN/2) block of
 //  16 x 16 MFMA tiles.  K advances in steps of BK through LDS (k-major tiles so that the
@@ -379,11 +396,12 @@ __device__ __forceinline__ void kron16_vec(const T (&ef)[4], const T (&ef2)[4], 
     for (int s = 0; s < 4; ++s) z = Mfma<T>::mma(ef2[s], y[s], z);
 }
 
-// Left product on the 16 x 16 lattice: each wave transforms 4 columns of X.
+// Left product on the 16 x 16 lattice: each wave transforms 4 columns of X, one after the other
+// through its own transpose buffer.
 template <typename T>
 __global__ __launch_bounds__(256) void kron_left16_kernel(KronArgs<T> a)
 {
-    __shared__ T zt[4][4][16][17];   // per wave and column: Z(i, j) at [j][i]
+    __shared__ T zt[4][16][17];   // per wave: Z(i, j) at [j][i]
     const int b = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     T ef[4], ef2[4];
     load_efrag<T>(a.E, lane, ef, ef2);
@@ -410,17 +428,17 @@ __global__ __launch_bounds__(256) void kron_left16_kernel(KronArgs<T> a)
         typename Mfma<T>::acc_t z;
         kron16_vec<T>(ef, ef2, vin[cc], z);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) zt[wave][cc][lane & 15][Mfma<T>::row(lane, r)] = z[r];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int cc = 0; cc < 4; ++cc) {
+        for (int r = 0; r < 4; ++r) zt[wave][lane & 15][Mfma<T>::row(lane, r)] = z[r];
+        wave_lds_sync();
         const int c = cbase + cc;
         const double cs = (a.cold.mode != DIAG_NONE) ? diag_value(a.cold, b, c) : 1.0;
+        T zv[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) zv[s] = zt[wave][(lane >> 4) + 4 * s][lane & 15];
+        wave_lds_sync();   // the buffer is read before the next column overwrites it
 #pragma unroll
         for (int s = 0; s < 4; ++s)   // site l + 64 s = i + 16 j with i = l%16, j = l/16 + 4 s
-            C[(long long)c * 256 + lane + 64 * s] =
-                T((double)zt[wave][cc][(lane >> 4) + 4 * s][lane & 15] * cs * dout[s]);
+            C[(long long)c * 256 + lane + 64 * s] = T((double)zv[s] * cs * dout[s]);
     }
 }
 
@@ -698,22 +716,6 @@ __device__ __forceinline__ void flush_mfma(T* __restrict__ G, int N, const T* __
     }
 }
 
-__device__ __forceinline__ double readlane_f64(double v, int l)
-{
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
-// LDS writes of some lanes of a wave become visible to the later LDS reads of the other lanes
-// of the same wave (no workgroup barrier).
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Accept/reject sequence of one site block S = b0 .. b0+bs-1 (bs <= 32) by one wave, with the
 // reference's rank-1 update applied to G(S,S) after every accepted flip.  Lane l holds row
