@@ -157,11 +157,10 @@ def main():
     # roofline: live per-launch timing (HIP events on the engine's stream) of the Metropolis
     # kernel, the path's largest single kernel, over two extra sweeps; the other timer classes
     # give the kernel-time shares
-    blocked = os.environ.get("DQMC_EXPERIMENTAL_KERNELS", "") == "1"
-    qr_name = "qr_kernel" if blocked else ("qr_v1_kernel" if os.environ.get("DQMC_QR", "") == "v1"
-                                           else "blocked QR (qr_panel4_kernel + qr_update_kernel per 32 columns)")
-    trsm_name = "trsm_kernel" if blocked else ("trsm_v1_kernel" if os.environ.get("DQMC_TRSM", "") == "v1"
-                                               else "blocked trsm (trsm_diag_kernel + gemm_kernel<double>)")
+    qr_name = ("qr_v1_kernel" if os.environ.get("DQMC_QR", "") == "v1"
+               else "blocked QR (qr_panel4_kernel + qr_update_kernel per 32 columns)")
+    trsm_name = ("trsm_v1_kernel" if os.environ.get("DQMC_TRSM", "") == "v1"
+                 else "blocked trsm (trsm_diag_kernel + gemm_kernel<double>)")
     kernel_names = {0: "metropolis_kernel<double, 1024>", 1: qr_name, 2: "gemm_kernel<double> (stack/recompose)",
                     3: trsm_name, 4: "kron_left/right_kernel<double>"}
     prof = {}

@@ -61,8 +61,6 @@ struct dqmc_ctx {
     double dt = 0, beta = 0, nnt = 0, u = 0, mu = 0, alpha = 0;
     bool eqtime = false, dynamic = false, thermalization = false, use_graphs = false;
     bool initialized = false;
-    // blocked QR / LDS triangular solve / Kronecker wraps: opt-in (DQMC_EXPERIMENTAL_KERNELS=1)
-    bool experimental = false;
     // QR of the stabilizations: 0 = blocked (panel kernel + MFMA trailing GEMMs), 1 = one-kernel
     // column-panel QR (DQMC_QR=v1)
     int qr_mode = 0;
@@ -107,6 +105,8 @@ struct dqmc_ctx {
     double *gt0A = nullptr, *gt0B = nullptr, *g0tA = nullptr, *g0tB = nullptr, *gt0 = nullptr, *g0t = nullptr;
     double *vecgftt = nullptr, *vecgft0 = nullptr, *vecgf0t = nullptr;
     double *wrap_eq = nullptr, *wrap_dyn = nullptr, *maxdG = nullptr;
+    uint32_t* health = nullptr;      // [B] HEALTH_* flags of the current call
+    double* wd_bound = nullptr;      // watchdog bound on max|G_recomputed - G_working| (0 = off)
     DStack left, right;
     // parameter adjoint at fixed fields (allocated on first use)
     int* nb = nullptr;
@@ -179,6 +179,11 @@ int kt_post(dqmc_ctx* c, int id)
 void lattice_expk(int nl, double x, std::vector<double>& out)
 {
     const int N = nl * nl;
+    if (x == 0.0) {   // nnt = 0: exactly the identity (the cosine sums below round to ~1e-17)
+        out.assign((size_t)N * N, 0.0);
+        for (int i = 0; i < N; ++i) out[i + (size_t)i * N] = 1.0;
+        return;
+    }
     std::vector<double> f(N, 0.0);  // value for displacement (dx, dy)
     for (int dy = 0; dy < nl; ++dy)
         for (int dx = 0; dx < nl; ++dx) {
@@ -204,6 +209,10 @@ void lattice_expk(int nl, double x, std::vector<double>& out)
 void ring_expk16(int nl, double x, std::vector<double>& out)
 {
     out.assign(256, 0.0);
+    if (x == 0.0) {   // exactly the identity
+        for (int i = 0; i < nl; ++i) out[i * 16 + i] = 1.0;
+        return;
+    }
     for (int i = 0; i < nl; ++i)
         for (int j = 0; j < nl; ++j) {
             double s = 0.0;
@@ -249,15 +258,13 @@ int gemm(dqmc_ctx* c, const T* A, long long sA, int lda, int tA, const T* B, lon
 
 hipError_t qr_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int batch, hipStream_t s)
 {
-    if (c->experimental) return launch_qr(buf, stride, N, W, batch, s);
     if (c->qr_mode == 1) return launch_qr_v1(buf, stride, N, W, batch, s);
     return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrT, c->qrWt, 3LL * N * 32, c->tile64, c->qr_update, s);
 }
 hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int c0, int P, int batch, hipStream_t s)
 {
-    if (!c->experimental && !c->trsm_v1) return launch_trsm_blocked(buf, stride, N, W, c0, P, batch, c->tile64, s);
-    return c->experimental ? launch_trsm_upper(buf, stride, N, W, c0, P, batch, s)
-                           : launch_trsm_upper_v1(buf, stride, N, W, c0, P, batch, s);
+    if (c->trsm_v1) return launch_trsm_upper_v1(buf, stride, N, W, c0, P, batch, s);
+    return launch_trsm_blocked(buf, stride, N, W, c0, P, batch, c->tile64, s);
 }
 
 // C = diag(r) (E(x)E) diag(s) X diag(cv)  (left) or  diag(r) X diag(s) (E(x)E) diag(cv)  (right)
@@ -362,14 +369,14 @@ int metropolis(dqmc_ctx* c, int t)
         a.G = c->G32; a.sG = c->NN();
         a.fields = c->fields; a.N = c->N; a.L = c->L; a.t = t;
         a.rng = c->rng; a.powtab = c->powtab; a.delta = c->delta;
-        a.accmask = c->accmask; a.acc_count = c->acc_count; a.kmax = c->kmax;
+        a.accmask = c->accmask; a.acc_count = c->acc_count; a.health = c->health; a.kmax = c->kmax;
         TIMED(c, KT_METRO, launch_metropolis<float>(a, c->B, c->metro_threads, c->stream));
     } else {
         MetroArgs<double> a;
         a.G = c->G64; a.sG = c->NN();
         a.fields = c->fields; a.N = c->N; a.L = c->L; a.t = t;
         a.rng = c->rng; a.powtab = c->powtab; a.delta = c->delta;
-        a.accmask = c->accmask; a.acc_count = c->acc_count; a.kmax = c->kmax;
+        a.accmask = c->accmask; a.acc_count = c->acc_count; a.health = c->health; a.kmax = c->kmax;
         TIMED(c, KT_METRO, launch_metropolis<double>(a, c->B, c->metro_threads, c->stream));
     }
     return 0;
@@ -395,7 +402,7 @@ int stack_push(dqmc_ctx* c, DStack& s)
     HCHECK(c, launch_udv_prepare(M, nn, c->qr, c->sqr(), c->perm, N, c->B, c->stream));
     TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
     HCHECK(c, launch_udv_finish(c->qr, c->sqr(), c->perm, sU(c, s, s.len), st, sD(c, s, s.len), sd, c->W2, nn, N,
-                                c->B, c->stream));
+                                c->health, c->B, c->stream));
     if (s.len == 0)
         HCHECK(c, launch_copy_d(c->W2, nn, sV(c, s, 0), st, nn, c->B, c->stream));
     else
@@ -506,7 +513,7 @@ int stabilize_eq(dqmc_ctx* c)
 {
     RCHECK(recompute_eq(c, c->W3));
     HCHECK(c, launch_compare_commit(c->W3, c->G64, c->prec == DQMC_MIXED ? c->G32 : nullptr, c->NN(), c->N, c->wrap_eq,
-                                    c->maxdG, c->B, c->stream));
+                                    c->maxdG, c->wd_bound, c->health, c->B, c->stream));
     return set_identity_stk(c);
 }
 
@@ -573,9 +580,9 @@ int do_sweep_dynamic(dqmc_ctx* c)
             RCHECK(stack_push(c, c->left));
             RCHECK(recompute_dyn(c, c->W3, c->W2, c->W1));
             HCHECK(c, launch_compare_commit(c->W3, c->G64, c->prec == DQMC_MIXED ? c->G32 : nullptr, nn, N, c->wrap_eq,
-                                            c->maxdG, c->B, c->stream));
-            HCHECK(c, launch_compare_only(c->W2, c->gt0, nn, N, c->wrap_dyn, c->B, c->stream));
-            HCHECK(c, launch_compare_only(c->W1, c->g0t, nn, N, c->wrap_dyn, c->B, c->stream));
+                                            c->maxdG, c->wd_bound, c->health, c->B, c->stream));
+            HCHECK(c, launch_compare_only(c->W2, c->gt0, nn, N, c->wrap_dyn, c->health, c->B, c->stream));
+            HCHECK(c, launch_compare_only(c->W1, c->g0t, nn, N, c->wrap_dyn, c->health, c->B, c->stream));
             HCHECK(c, launch_copy_d(c->W2, nn, c->gt0, nn, nn, c->B, c->stream));
             HCHECK(c, launch_copy_d(c->W1, nn, c->g0t, nn, nn, c->B, c->stream));
             RCHECK(set_identity_stk(c));
@@ -609,9 +616,36 @@ int build_from_fields(dqmc_ctx* c)
         }
     }
     RCHECK(recompute_eq(c, c->W3));
+    HCHECK(c, launch_finite_check(c->W3, nn, c->N, c->health, c->B, c->stream));
     HCHECK(c, launch_copy_d(c->W3, nn, c->G64, nn, nn, c->B, c->stream));
     if (c->prec == DQMC_MIXED) HCHECK(c, launch_copy_d2f(c->W3, nn, c->G32, nn, nn, c->B, c->stream));
     return 0;
+}
+
+// Numeric health of the chains over one ABI call: cleared before the call's work, read after it.
+int health_reset(dqmc_ctx* c)
+{
+    HCHECK(c, launch_fill_u32(c->health, c->B, 0u, c->stream));
+    return 0;
+}
+const char* health_text(uint32_t h)
+{
+    if (h & HEALTH_NONFINITE_G) return "non-finite Green's function at a stabilization";
+    if (h & HEALTH_ZERO_PIVOT) return "zero or non-finite pivot in a UDV factorization";
+    if (h & HEALTH_RATIO) return "non-finite acceptance ratio";
+    return "wrap error above the watchdog bound";
+}
+int health_check(dqmc_ctx* c, const char* what)
+{
+    std::vector<uint32_t> h(c->B);
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(h.data(), c->health, (size_t)c->B * 4, hipMemcpyDeviceToHost));
+    int bad = 0, first = -1;
+    for (int b = 0; b < c->B; ++b)
+        if (h[b]) { ++bad; if (first < 0) first = b; }
+    if (!bad) return 0;
+    return c->fail(DQMC_ENUMERIC, "%s: chain %d: %s (flags 0x%x; %d chain(s) flagged)", what, first,
+                   health_text(h[first]), h[first], bad);
 }
 
 // Buffers of the fixed-field forward and the adjoint: the dynamic G vectors and the
@@ -770,14 +804,20 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     c->alpha = std::acosh(std::exp(0.5 * c->dt * c->u));   // src/hubbard.cpp:35
     c->eqtime = p->collect_eqtime != 0; c->dynamic = p->collect_dynamic != 0;
     {
-        const char* ex = std::getenv("DQMC_EXPERIMENTAL_KERNELS");
-        c->experimental = ex && ex[0] == '1';
         const char* q = std::getenv("DQMC_QR");
         c->qr_mode = (q && std::string(q) == "v1") ? 1 : 0;
         const char* qu = std::getenv("DQMC_QR_UPDATE");
         if (qu && std::string(qu) == "gemm") c->qr_update = QR_UPDATE_GEMM;
         const char* mt = std::getenv("DQMC_METRO_THREADS");
-        if (mt) c->metro_threads = std::atoi(mt);
+        if (mt) {
+            const int v = std::atoi(mt);
+            // the pipelined flush needs waves beside the site-loop wave: 256, 512 or 1024 threads
+            if (v != 256 && v != 512 && v != 1024) {
+                delete c;
+                return DQMC_EINVAL;
+            }
+            c->metro_threads = v;
+        }
         const char* tr = std::getenv("DQMC_TRSM");
         c->trsm_v1 = tr && std::string(tr) == "v1";
         const char* w = std::getenv("DQMC_WRAP");
@@ -790,7 +830,7 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     const int B = n_chains, L = c->L, S = c->S;
     const long long nn = c->NN();
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return c->fail(DQMC_EHIP, "stream");
-    HCHECK(c, prepare_kernels(c->experimental));
+    HCHECK(c, prepare_kernels());
     for (auto& e : c->ev) HCHECK(c, hipEventCreate(&e));
     RCHECK(c->alloc(&c->expK, nn * 8)); RCHECK(c->alloc(&c->invExpK, nn * 8));
     RCHECK(c->alloc(&c->E1, 256 * 8)); RCHECK(c->alloc(&c->F1, 256 * 8));
@@ -826,6 +866,7 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     }
     RCHECK(c->alloc(&c->wrap_eq, (size_t)B * 8)); RCHECK(c->alloc(&c->wrap_dyn, (size_t)B * 8));
     RCHECK(c->alloc(&c->maxdG, (size_t)B * 8));
+    RCHECK(c->alloc(&c->health, (size_t)B * 4)); RCHECK(c->alloc(&c->wd_bound, 8));
 
     // constant tables
     std::vector<double> ek, iek;
@@ -861,6 +902,10 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     HCHECK(c, hipMemcpy(c->fields, ones.data(), ones.size(), hipMemcpyHostToDevice));
     std::vector<uint64_t> r1(B, 1ULL);
     HCHECK(c, hipMemcpy(c->rng, r1.data(), (size_t)B * 8, hipMemcpyHostToDevice));
+    const double off = 0.0;
+    HCHECK(c, hipMemcpy(c->wd_bound, &off, 8, hipMemcpyHostToDevice));
+    std::vector<uint32_t> h0(B, 0u);
+    HCHECK(c, hipMemcpy(c->health, h0.data(), (size_t)B * 4, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -981,11 +1026,12 @@ int dqmc_init(dqmc_ctx* c)
     HCHECK(c, launch_fill_d(c->wrap_dyn, B, 0.0, c->stream));
     HCHECK(c, launch_fill_d(c->maxdG, B, 0.0, c->stream));
     if (!c->initialized) RCHECK(autotune(c));
+    RCHECK(health_reset(c));
     RCHECK(build_from_fields(c));
     HCHECK(c, hipStreamSynchronize(c->stream));
     (void)N; (void)L; (void)nn;
     c->initialized = true;
-    return 0;
+    return health_check(c, "dqmc_init");
 }
 
 int dqmc_apply_bop(dqmc_ctx* c, int chain, int op, int t, double* inout)
@@ -1020,10 +1066,11 @@ int dqmc_sweep_up(dqmc_ctx* c)
     if (!c) return DQMC_EINVAL;
     if (!c->initialized || c->current_t != 0 || c->left.len != 0 || c->right.len != c->S)
         return c->fail(DQMC_ESTATE, "sweep_up needs current_t == 0, empty left and full right stacks");
+    RCHECK(health_reset(c));
     RCHECK(run_sweep(c, do_sweep_up, 0));
     c->current_t = c->L;
     c->updating_sweeps += 1;
-    return 0;
+    return health_check(c, "dqmc_sweep_up");
 }
 
 int dqmc_sweep_down(dqmc_ctx* c)
@@ -1031,10 +1078,11 @@ int dqmc_sweep_down(dqmc_ctx* c)
     if (!c) return DQMC_EINVAL;
     if (!c->initialized || c->current_t != c->L || c->right.len != 0 || c->left.len != c->S)
         return c->fail(DQMC_ESTATE, "sweep_down needs current_t == nt, empty right and full left stacks");
+    RCHECK(health_reset(c));
     RCHECK(run_sweep(c, do_sweep_down, 1));
     c->current_t = 0;
     c->updating_sweeps += 1;
-    return 0;
+    return health_check(c, "dqmc_sweep_down");
 }
 
 int dqmc_sweep_dynamic(dqmc_ctx* c)
@@ -1043,9 +1091,10 @@ int dqmc_sweep_dynamic(dqmc_ctx* c)
     if (!c->dynamic) return 0;   // Core::sweep_for_dynamic_green_functions is a no-op then
     if (!c->initialized || c->current_t != 0 || c->left.len != 0 || c->right.len != c->S)
         return c->fail(DQMC_ESTATE, "sweep_dynamic needs current_t == 0, empty left and full right stacks");
+    RCHECK(health_reset(c));
     RCHECK(run_sweep(c, do_sweep_dynamic, 2));
     c->current_t = c->L;
-    return 0;
+    return health_check(c, "dqmc_sweep_dynamic");
 }
 
 int dqmc_get_gf(dqmc_ctx* c, int chain, int kind, int t, double* out)
@@ -1204,6 +1253,7 @@ int dqmc_forward_fixed(dqmc_ctx* c, double* logdet, double* gdiag)
     RCHECK(ensure_adjoint_buffers(c));
     SavedStats st;
     RCHECK(save_stats(c, st));
+    RCHECK(health_reset(c));
     RCHECK(build_from_fields(c));
     HCHECK(c, launch_logdet_00bb(c->qr, c->sqr(), 2 * N, sD(c, c->right, c->right.len - 1), (long long)c->S * N, N,
                                  c->adj_out, B, c->stream));
@@ -1213,7 +1263,8 @@ int dqmc_forward_fixed(dqmc_ctx* c, double* logdet, double* gdiag)
     if (logdet) HCHECK(c, hipMemcpy(logdet, c->adj_out, (size_t)B * 8, hipMemcpyDeviceToHost));
     if (gdiag) HCHECK(c, hipMemcpy(gdiag, c->adj_gdiag, (size_t)B * L * N * 8, hipMemcpyDeviceToHost));
     RCHECK(build_from_fields(c));
-    return restore_stats(c, st);
+    RCHECK(restore_stats(c, st));
+    return health_check(c, "dqmc_forward_fixed");
 }
 
 int dqmc_adjoint(dqmc_ctx* c, const double* gbar_logdet, const double* gbar_gdiag, double* grad)
@@ -1232,6 +1283,7 @@ int dqmc_adjoint(dqmc_ctx* c, const double* gbar_logdet, const double* gbar_gdia
     if (gbar_gdiag) HCHECK(c, hipMemcpy(c->adj_gdiag, gbar_gdiag, (size_t)B * L * N * 8, hipMemcpyHostToDevice));
     else HCHECK(c, launch_fill_d(c->adj_gdiag, (long long)B * L * N, 0.0, c->stream));
     HCHECK(c, launch_fill_d(c->adj_grad, (long long)B * 3, 0.0, c->stream));
+    RCHECK(health_reset(c));
     HCHECK(c, hipMemcpyAsync(c->fields_save, c->fields, (size_t)B * L * N, hipMemcpyDeviceToDevice, c->stream));
     // d alpha / d u for cosh(alpha) = e^{dt u / 2} (src/hubbard.cpp:35)
     const double dalpha = c->dt * std::exp(0.5 * c->dt * c->u) / (2.0 * std::sinh(c->alpha));
@@ -1251,7 +1303,26 @@ int dqmc_adjoint(dqmc_ctx* c, const double* gbar_logdet, const double* gbar_gdia
     RCHECK(build_from_fields(c));
     HCHECK(c, hipStreamSynchronize(c->stream));
     HCHECK(c, hipMemcpy(grad, c->adj_grad, (size_t)B * 3 * 8, hipMemcpyDeviceToHost));
-    return restore_stats(c, st);
+    RCHECK(restore_stats(c, st));
+    return health_check(c, "dqmc_adjoint");
+}
+
+int dqmc_set_watchdog(dqmc_ctx* c, double max_dG)
+{
+    if (!c) return DQMC_EINVAL;
+    if (!(max_dG >= 0.0)) return c->fail(DQMC_EINVAL, "dqmc_set_watchdog: bound must be >= 0 (0 = off)");
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(c->wd_bound, &max_dG, 8, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int dqmc_get_health(dqmc_ctx* c, int chain, uint32_t* flags)
+{
+    RCHECK(check_chain(c, chain));
+    if (!flags) return DQMC_EINVAL;
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(flags, c->health + chain, 4, hipMemcpyDeviceToHost));
+    return 0;
 }
 
 int dqmc_dims(dqmc_ctx* c, int* ns, int* nt, int* nch)

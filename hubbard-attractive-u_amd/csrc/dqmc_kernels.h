@@ -12,6 +12,14 @@
 
 namespace dqmc {
 
+// Per-chain numeric health flags, OR-ed in by the kernels that detect them.
+enum HealthFlag : uint32_t {
+    HEALTH_NONFINITE_G = 1u,   // NaN/Inf in the working G or its FP64 recompute at a stabilization
+    HEALTH_ZERO_PIVOT = 2u,    // zero or non-finite |R_ii| in a UDV factorization
+    HEALTH_RATIO = 4u,         // non-finite acceptance ratio (non-finite G_ii)
+    HEALTH_WATCHDOG = 8u       // max |G_recomputed - G_working| above the watchdog bound
+};
+
 // Diagonal scaling operand of a GEMM: value for index i of chain b.
 enum DiagMode : int {
     DIAG_NONE = 0,
@@ -77,6 +85,7 @@ struct MetroArgs {
     const double* delta;              // [chain][2]: e^{-2 alpha s} - 1 for s = +1, -1
     uint32_t* accmask;                // [chain][L][ceil(N/32)]
     unsigned long long* acc_count;    // [chain]
+    uint32_t* health;                 // [chain] HEALTH_* flags
     int kmax;
 };
 // threads: workgroup size for N >= 256 (256, 512 or 1024); smaller lattices use 2 x N rounded
@@ -86,14 +95,11 @@ hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, int threads, hipS
 // LDS bytes of one Metropolis workgroup (krows pending-update rows of X and Y).
 size_t metropolis_lds_bytes(int N, int krows, size_t esz);
 // One-time kernel attribute setup (large dynamic LDS); call before any graph capture.
-hipError_t prepare_kernels(bool blocked_qr);
+hipError_t prepare_kernels();
 
 // Householder QR (no pivoting) of the leading N x N block of a row-major N x W buffer,
 // applying Q^T to the trailing W-N columns.  On exit the leading block holds R (zeros
-// below the diagonal).
-hipError_t launch_qr(double* buf, long long stride, int N, int W, int batch, hipStream_t s);
-
-// Same contracts, reference-path implementations (see the kernel file).
+// below the diagonal).  One-kernel form (DQMC_QR=v1).
 hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, hipStream_t s);
 // Blocked Householder QR of the same row-major N x W system (N <= 256): per panel of 32 columns
 // one factorization launch (one workgroup per chain) and the trailing update of the remaining
@@ -109,13 +115,10 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
 // right-hand side) and MFMA GEMM updates of the rows above (`tile` = GEMM tile).
 hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,
                                hipStream_t s);
+// In-place X = R^{-1} Z, R the upper triangle of the leading N x N block, Z columns
+// [c0, c0+P) of the same row-major N x W buffer; one-kernel form (DQMC_TRSM=v1).
 hipError_t launch_trsm_upper_v1(double* buf, long long stride, int N, int W, int c0, int P, int batch,
                                 hipStream_t s);
-
-// In-place X = R^{-1} Z, R the upper triangle of the leading N x N block, Z columns
-// [c0, c0+P) of the same row-major N x W buffer.
-hipError_t launch_trsm_upper(double* buf, long long stride, int N, int W, int c0, int P, int batch,
-                             hipStream_t s);
 
 // Pre-pivoted UDV input: M column-major (ld N); writes buf[r][j] = M(r, perm[j]) with perm
 // sorting the columns of M by descending 2-norm, and buf[r][N+j] = delta_rj.
@@ -124,7 +127,7 @@ hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long l
 // After QR: D_i = |R_ii|, Tt (column-major N x N) with Tt(perm[j], i) = R_ij / D_i (j >= i),
 // U (column-major) = Q read from the trailing block.
 hipError_t launch_udv_finish(const double* buf, long long sbuf, const int* perm, double* U, long long sU,
-                             double* D, long long sD, double* Tt, long long sTt, int N, int batch,
+                             double* D, long long sD, double* Tt, long long sTt, int N, uint32_t* health, int batch,
                              hipStream_t s);
 
 // Builds the row-major system for (1 + u diag(d) v^T)^{-1}:
@@ -142,15 +145,19 @@ hipError_t launch_build_rhs(const double* X, long long sX, DiagArg d, double alp
 hipError_t launch_extract(const double* buf, long long sbuf, int W, int c0, double* out, long long so, int N,
                           int transpose, int batch, hipStream_t s);
 
-// Signed-max wrap error and max-abs difference of a new FP64 G against the working copy;
-// then commits new -> G64 (and G32 if non-null).  err_signed / err_abs are per-chain
-// running maxima (updated in place).  sign_mode: 0 = plain, 1 = gt0 = I - g, 2 = g0t = -g, etc. are
-// handled by callers.
+// Signed-max wrap error (max_ij (Gnew - G)_ij, include/utils/stable_numerics.hpp:42) and max-abs
+// difference of a new FP64 G against the working copy; then commits new -> G64 (and G32 if
+// non-null).  err_signed / err_abs are per-chain running maxima (updated in place; the caller
+// starts them at 0 like src/dqmc_core.cpp:211-212).  Sets HEALTH_NONFINITE_G for a non-finite
+// entry on either side and HEALTH_WATCHDOG when *bound > 0 and the max-abs difference exceeds it.
 hipError_t launch_compare_commit(const double* Gnew, double* G64, float* G32, long long sG, int N,
-                                 double* err_signed, double* err_abs, int batch, hipStream_t s);
-// Same but the working matrix is FP64 only (dynamic G(t,0), G(0,t)).
+                                 double* err_signed, double* err_abs, const double* bound, uint32_t* health,
+                                 int batch, hipStream_t s);
+// Same but the working matrix is FP64 only (dynamic G(t,0), G(0,t)); no commit.
 hipError_t launch_compare_only(const double* Gnew, const double* Gcur, long long sG, int N, double* err_signed,
-                               int batch, hipStream_t s);
+                               uint32_t* health, int batch, hipStream_t s);
+// HEALTH_NONFINITE_G for chains whose N x N matrix holds a non-finite entry.
+hipError_t launch_finite_check(const double* A, long long sA, int N, uint32_t* health, int batch, hipStream_t s);
 
 // Elementwise helpers (n elements per chain).
 hipError_t launch_set_identity(double* A, long long s, int N, double diagval, int batch, hipStream_t st);
@@ -164,6 +171,7 @@ hipError_t launch_copy_d2f(const double* src, long long ss, float* dst, long lon
 hipError_t launch_axpi(const double* src, long long ss, double* dst, long long sd, double a, double b, int N,
                        int batch, hipStream_t st);
 hipError_t launch_fill_d(double* dst, long long n, double v, hipStream_t st);
+hipError_t launch_fill_u32(uint32_t* dst, long long n, uint32_t v, hipStream_t st);
 
 // ---- parameter adjoint at fixed fields (dqmc_adjoint.hip) ----------------------------------
 struct AdjointArgs {

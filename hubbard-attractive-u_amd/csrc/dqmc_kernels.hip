@@ -6,9 +6,9 @@
 //   metropolis_kernel .. one workgroup per chain walks the N sites of a slice: FP64 ratio
 //                        from G_ii, minstd_rand0 uniforms by jump-ahead, delayed rank-k
 //                        Sherman-Morrison updates flushed into G inside the workgroup.
-//   qr_kernel .......... blocked Householder QR (panels in LDS) of a row-major N x N block,
-//                        applying Q^T to trailing right-hand-side columns.
-//   trsm_kernel ........ X = R^{-1} Z, row blocks of 16 held in registers.
+//   qr_panel4_kernel ... blocked Householder QR of a row-major N x N block (panel in registers,
+//   qr_update_kernel     trailing update on MFMA), applying Q^T to right-hand-side columns.
+//   trsm_diag_kernel ... X = R^{-1} Z, 32-row diagonal blocks; the updates above run on MFMA.
 //   small helpers ...... UDV pre-pivot / finish, system builders, error reductions, copies.
 #include "dqmc_kernels.h"
 
@@ -591,7 +591,7 @@ struct MetroLds {
     double* gam;   // [kb]
     int* sl;       // [N] fields of the slice
     int* ql;       // [kb] block-local index of accepted flip m
-    int* nacc;     // [1] accepted flips of the current block
+    int* nacc;     // [2] accepted flips of site block j at nacc[j & 1]
     uint32_t* mask;
 };
 
@@ -611,7 +611,7 @@ __device__ MetroLds<T> metro_lds(unsigned char* smem, int N, int kb)
     L.sl = reinterpret_cast<int*>(L.gam + kb);
     L.ql = L.sl + N;
     L.nacc = L.ql + kb;
-    L.mask = reinterpret_cast<uint32_t*>(L.nacc + 1);
+    L.mask = reinterpret_cast<uint32_t*>(L.nacc + 2);
     return L;
 }
 
@@ -621,7 +621,7 @@ size_t metropolis_lds_bytes(int N, int kb, size_t esz)
     size_t b = (size_t)2 * kb4 * N * esz;
     b = (b + 7) / 8 * 8;
     b += (size_t)8 * (64 + 2 * (size_t)kb * kb + N + kb);
-    b += (size_t)4 * (N + kb + 1) + (size_t)4 * ((N + 31) / 32) + 64;
+    b += (size_t)4 * (N + kb + 2) + (size_t)4 * ((N + 31) / 32) + 64;
     return b;
 }
 
@@ -725,10 +725,10 @@ __device__ __forceinline__ void flush_mfma(T* __restrict__ G, int N, const T* __
 // is accepted, so every lane recomputes them then (in parallel), and a rejected site costs two
 // lane reads and a compare.  An accepted flip q publishes column q and row q through LDS; every
 // lane then applies G(r, c) += (gamma G(r, q)) (G(q, c) - delta_qc).  Records XS, YS, gam, ql and
-// the flip count for the rebuild, and flips the fields.
+// the flip count (in nacc[(b0 / kb) & 1]) for the rebuild, and flips the fields.
 template <typename T>
 __device__ __forceinline__ void metro_site_block(const MetroLds<T>& L, const T* __restrict__ G, int8_t* f, int N,
-                                                 int kb, int b0, int bs, double dpos, double dneg)
+                                                 int kb, int b0, int bs, double dpos, double dneg, uint32_t* health)
 {
     const int lane = threadIdx.x & 63, r = lane & 31, h = lane >> 5;
     double g[16];
@@ -742,6 +742,9 @@ __device__ __forceinline__ void metro_site_block(const MetroLds<T>& L, const T* 
     const int sr = (r < bs) ? L.sl[b0 + r] : 1;
     const double dr = sr > 0 ? dpos : dneg;
     double xr, pr = accept_prob(dg, dr, &xr), gr = dr / xr;
+    // A non-finite G_ii gives a non-finite ratio (the reference would take min(1, NaN) as 1);
+    // within the block G(S,S) stays finite when it starts finite, so the block start is checked.
+    if (__any(r < bs && !isfinite(xr)) && lane == 0) atomicOr(health, HEALTH_RATIO);
     int na = 0;
     for (int qh = 0; qh < 2; ++qh) {
         if (16 * qh >= bs) break;
@@ -786,7 +789,7 @@ __device__ __forceinline__ void metro_site_block(const MetroLds<T>& L, const T* 
             }
         }
     }
-    if (lane == 0) *L.nacc = na;
+    if (lane == 0) L.nacc[(b0 / kb) & 1] = na;
 }
 
 // Submatrix Metropolis update of slice t.  Sites are visited in blocks S of kb consecutive
@@ -825,11 +828,14 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
     // cross of rows and columns of block B+1 (all waves); then wave 0 runs block B+1's site loop,
     // which reads only that cross, while the other waves flush the rest of G.
     __syncthreads();
-    if (tid < 64) metro_site_block(L, G, f, N, kb, 0, min(kb, N), dpos, dneg);
+    uint32_t* health = a.health + b;
+    if (tid < 64) metro_site_block(L, G, f, N, kb, 0, min(kb, N), dpos, dneg, health);
     __syncthreads();
     for (int b0 = 0; b0 < N; b0 += kb) {
         const int bs = min(kb, N - b0);
-        const int na = *L.nacc;
+        // The count of block b0 sits in its own parity slot: when na == 0 no barrier separates this
+        // read from the site loop of the next block, which writes the other slot.
+        const int na = L.nacc[(b0 / kb) & 1];
         nacc += na;
         const int b1 = b0 + kb, bs1 = (b1 < N) ? min(kb, N - b1) : 0;
         const bool next = b1 < N;
@@ -893,7 +899,7 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
         }
         if (next) {
             if (tid < 64)
-                metro_site_block(L, G, f, N, kb, b1, bs1, dpos, dneg);
+                metro_site_block(L, G, f, N, kb, b1, bs1, dpos, dneg, health);
             else if (na > 0)
                 flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, ((na + 3) / 4) * 4, FLUSH_REST, tr0, tr1, 1);
         } else if (na > 0) {
@@ -910,10 +916,9 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
     }
 }
 
-__global__ void qr_kernel(double* __restrict__ bufs, long long stride, int N, int W);
 
 
-hipError_t prepare_kernels(bool blocked_qr)
+hipError_t prepare_kernels()
 {
     const void* ks[] = {(const void*)metropolis_kernel<double, 256>, (const void*)metropolis_kernel<double, 512>,
                         (const void*)metropolis_kernel<double, 1024>, (const void*)metropolis_kernel<float, 256>,
@@ -922,8 +927,6 @@ hipError_t prepare_kernels(bool blocked_qr)
         hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         if (e != hipSuccess) return e;
     }
-    if (blocked_qr)
-        return hipFuncSetAttribute((const void*)qr_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
     return hipSuccess;
 }
 
@@ -946,14 +949,10 @@ template hipError_t launch_metropolis<double>(const MetroArgs<double>&, int, int
 template hipError_t launch_metropolis<float>(const MetroArgs<float>&, int, int, hipStream_t);
 
 // ---------------------------------------------------------------------------------------------
-//  Blocked Householder QR of a row-major N x N block with Q^T applied to W-N trailing columns.
-//  One 256-thread workgroup per chain.  Panel of NB columns lives in LDS; its reflectors
-//  (LAPACK dlarfg convention) form the compact WY factor T; the trailing update
-//  A <- (I - V T^T V^T) A runs thread-per-column (coalesced along rows of the row-major buffer).
+//  Cross-lane reductions shared by the QR kernels.
 // ---------------------------------------------------------------------------------------------
-#define QR_NB 16
+#define QR_NB 16      // panel width of the one-kernel QR (qr_v1_kernel)
 #define QR_MAXN 256
-#define QR_CH 8    // rows per register chunk in the trailing update
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -1009,236 +1008,11 @@ __device__ __forceinline__ double row_sum_dpp(double v)
 }
 __device__ __forceinline__ double wave_sum_dpp(double v) { return sum_xor32(sum_xor16(row_sum_dpp(v))); }
 
-// Blocked Householder QR.  Per panel of QR_NB columns:
-//   1. the panel (rows p0..N-1) is factored in LDS, one reflector per column (dlarfg);
-//   2. its R rows go back to global memory, the reflectors V (unit lower trapezoid, explicit
-//      ones and zeros) stay in LDS together with the compact WY factor T;
-//   3. every trailing column (thread-per-column, coalesced along the row-major rows) is
-//      updated as a <- a - V T^T V^T a, streaming the column through registers in chunks of
-//      QR_CH rows so that QR_CH independent loads are in flight.
-struct QrLds {
-    double Pv[QR_MAXN + QR_CH][QR_NB];   // panel, then the reflectors V (unit lower trapezoid)
-    double Yv[QR_MAXN + QR_CH][QR_NB];   // V T^T: the trailing update is a <- a - Yv (V^T a)
-    double Tm[QR_NB][QR_NB];
-    double VtV[QR_NB][QR_NB];
-    double tau[QR_NB];
-};
 
-__global__ __launch_bounds__(256) void qr_kernel(double* __restrict__ bufs, long long stride, int N, int W)
-{
-    extern __shared__ __align__(16) unsigned char qr_smem[];
-    QrLds& S = *reinterpret_cast<QrLds*>(qr_smem);
-    auto& Pv = S.Pv;
-    auto& Yv = S.Yv;
-    auto& Tm = S.Tm;
-    auto& VtV = S.VtV;
-    auto& tau = S.tau;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    double* A = bufs + (long long)blockIdx.x * stride;
-
-    for (int p0 = 0; p0 < N; p0 += QR_NB) {
-        const int nb = min(QR_NB, N - p0);
-        const int R = N - p0;
-        // rows up to the next multiple of QR_CH are zero-filled: the chunked trailing update
-        // reads them (multiplied by zero) past the last row
-        const int Rp = ((R + QR_CH - 1) / QR_CH) * QR_CH;
-        for (int e = tid; e < Rp * QR_NB; e += 256) {
-            const int r = e / QR_NB, c = e % QR_NB;
-            Pv[r][c] = (c < nb && r < R) ? A[(long long)(p0 + r) * W + p0 + c] : 0.0;
-        }
-        __syncthreads();
-        for (int c = 0; c < nb; ++c) {
-            if (wave == 0) {
-                const double alpha = Pv[c][c];   // read by every lane before lane 0 overwrites it
-                double s = 0.0;
-                for (int r = c + 1 + lane; r < R; r += 64) s += Pv[r][c] * Pv[r][c];
-                s = wave_sum(s);
-                double tc = 0.0, beta = alpha, scal = 0.0;
-                if (s > 0.0) {
-                    beta = -copysign(sqrt(alpha * alpha + s), alpha);
-                    tc = (beta - alpha) / beta;
-                    scal = 1.0 / (alpha - beta);
-                }
-                for (int r = c + 1 + lane; r < R; r += 64) Pv[r][c] *= scal;
-                if (lane == 0) { Pv[c][c] = beta; tau[c] = tc; }
-            }
-            __syncthreads();
-            for (int cc = c + 1 + wave; cc < nb; cc += 4) {
-                double s = (lane == 0) ? Pv[c][cc] : 0.0;
-                for (int r = c + 1 + lane; r < R; r += 64) s += Pv[r][c] * Pv[r][cc];
-                s = wave_sum(s) * tau[c];
-                if (lane == 0) Pv[c][cc] -= s;
-                for (int r = c + 1 + lane; r < R; r += 64) Pv[r][cc] -= s * Pv[r][c];
-            }
-            __syncthreads();
-        }
-        // R rows of the panel to global memory (below-diagonal entries are never read)
-        for (int e = tid; e < nb * nb; e += 256) {
-            const int r = e / nb, c = e % nb;
-            if (r <= c) A[(long long)(p0 + r) * W + p0 + c] = Pv[r][c];
-        }
-        __syncthreads();
-        // make V explicit: ones on the diagonal, zeros above
-        for (int e = tid; e < QR_NB * QR_NB; e += 256) {
-            const int r = e / QR_NB, c = e % QR_NB;
-            if (r <= c) Pv[r][c] = (r == c && c < nb) ? 1.0 : 0.0;
-        }
-        __syncthreads();
-        for (int e = tid; e < nb * nb; e += 256) {
-            const int i = e / nb, j = e % nb;
-            double s = 0.0;
-            if (i < j)
-                for (int r = j; r < R; ++r) s += Pv[r][i] * Pv[r][j];
-            VtV[i][j] = s;
-        }
-        __syncthreads();
-        if (tid == 0) {
-            for (int i = 0; i < QR_NB; ++i) {
-                for (int r = 0; r < QR_NB; ++r) Tm[r][i] = 0.0;
-                if (i >= nb) continue;
-                for (int r = 0; r < i; ++r) {
-                    double s = 0.0;
-                    for (int q = r; q < i; ++q) s += Tm[r][q] * VtV[q][i];
-                    Tm[r][i] = -tau[i] * s;
-                }
-                Tm[i][i] = tau[i];
-            }
-        }
-        __syncthreads();
-        // Yv = V T^T : Yv[r][m] = sum_{q >= m} V[r][q] T[m][q]
-        for (int e = tid; e < R * QR_NB; e += 256) {
-            const int r = e / QR_NB, m = e % QR_NB;
-            double acc = 0.0;
-            for (int q = m; q < QR_NB; ++q) acc += Pv[r][q] * Tm[m][q];
-            Yv[r][m] = acc;
-        }
-        for (int e = R * QR_NB + tid; e < ((R + QR_CH - 1) / QR_CH) * QR_CH * QR_NB; e += 256)
-            Yv[e / QR_NB][e % QR_NB] = 0.0;
-        __syncthreads();
-        for (int col = p0 + nb + tid; col < W; col += 256) {
-            double* Acol = A + (long long)p0 * W + col;
-            double w[QR_NB];
-#pragma unroll
-            for (int m = 0; m < QR_NB; ++m) w[m] = 0.0;
-            for (int r0 = 0; r0 < R; r0 += QR_CH) {
-                double x[QR_CH];
-#pragma unroll
-                for (int i = 0; i < QR_CH; ++i) x[i] = (r0 + i < R) ? Acol[(long long)(r0 + i) * W] : 0.0;
-#pragma unroll
-                for (int m2 = 0; m2 < QR_NB / 2; ++m2) {
-#pragma unroll
-                    for (int i = 0; i < QR_CH; ++i) {
-                        const double2 v = reinterpret_cast<const double2*>(&Pv[r0 + i][0])[m2];
-                        w[2 * m2] += v.x * x[i];
-                        w[2 * m2 + 1] += v.y * x[i];
-                    }
-                }
-            }
-            for (int r0 = 0; r0 < R; r0 += QR_CH) {
-                double x[QR_CH];
-#pragma unroll
-                for (int i = 0; i < QR_CH; ++i) x[i] = (r0 + i < R) ? Acol[(long long)(r0 + i) * W] : 0.0;
-#pragma unroll
-                for (int m2 = 0; m2 < QR_NB / 2; ++m2) {
-#pragma unroll
-                    for (int i = 0; i < QR_CH; ++i) {
-                        const double2 v = reinterpret_cast<const double2*>(&Yv[r0 + i][0])[m2];
-                        x[i] -= v.x * w[2 * m2] + v.y * w[2 * m2 + 1];
-                    }
-                }
-#pragma unroll
-                for (int i = 0; i < QR_CH; ++i)
-                    if (r0 + i < R) Acol[(long long)(r0 + i) * W] = x[i];
-            }
-        }
-        __syncthreads();
-    }
-}
-
-hipError_t launch_qr(double* buf, long long stride, int N, int W, int batch, hipStream_t s)
-{
-    if (N > QR_MAXN) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(qr_kernel, dim3(batch), dim3(256), sizeof(QrLds), s, buf, stride, N, W);
-    return hipGetLastError();
-}
 
 // ---------------------------------------------------------------------------------------------
-//  Upper triangular solve X = R^{-1} Z, thread-per-right-hand-side column.  Row blocks of
-//  TRSM_BR from the bottom: the diagonal block is solved in registers, then the rows above are
-//  updated with the block column R[0:i0, i0:i0+BR) staged in LDS (broadcast reads) while each
-//  thread streams its column in chunks of TRSM_BR rows.
-// ---------------------------------------------------------------------------------------------
-#define TRSM_BR 16
-#define TRSM_CH 8
-__global__ __launch_bounds__(256) void trsm_kernel(double* __restrict__ bufs, long long stride, int N, int W,
-                                                   int c0, int P)
-{
-    __shared__ __align__(16) double Rb[QR_MAXN][TRSM_BR];
-    double* A = bufs + (long long)blockIdx.y * stride;
-    const int col = blockIdx.x * 256 + threadIdx.x;
-    const bool active = col < P;
-    double* Z = A + c0 + (active ? col : 0);
-    for (int iend = N; iend > 0; iend -= TRSM_BR) {
-        const int i0 = iend - TRSM_BR > 0 ? iend - TRSM_BR : 0;
-        const int bs = iend - i0;
-        __syncthreads();
-        for (int e = threadIdx.x; e < iend * TRSM_BR; e += 256) {
-            const int k = e / TRSM_BR, m = e % TRSM_BR;
-            Rb[k][m] = (m < bs) ? A[(long long)k * W + i0 + m] : 0.0;
-        }
-        __syncthreads();
-        if (!active) continue;
-        double x[TRSM_BR];
-#pragma unroll
-        for (int m = 0; m < TRSM_BR; ++m) x[m] = (m < bs) ? Z[(long long)(i0 + m) * W] : 0.0;
-#pragma unroll
-        for (int m = TRSM_BR - 1; m >= 0; --m) {
-            if (m < bs) {
-                double s = x[m];
-#pragma unroll
-                for (int q = m + 1; q < TRSM_BR; ++q) s -= Rb[i0 + m][q] * x[q];
-                x[m] = s / Rb[i0 + m][m];
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < TRSM_BR; ++m)
-            if (m < bs) Z[(long long)(i0 + m) * W] = x[m];
-        for (int k0 = 0; k0 < i0; k0 += TRSM_CH) {
-            double z[TRSM_CH];
-#pragma unroll
-            for (int i = 0; i < TRSM_CH; ++i) z[i] = (k0 + i < i0) ? Z[(long long)(k0 + i) * W] : 0.0;
-#pragma unroll
-            for (int i = 0; i < TRSM_CH; ++i) {
-                const int k = min(k0 + i, QR_MAXN - 1);
-                const double2* rr = reinterpret_cast<const double2*>(&Rb[k][0]);
-                double s = z[i];
-#pragma unroll
-                for (int m2 = 0; m2 < TRSM_BR / 2; ++m2) {
-                    const double2 v = rr[m2];
-                    s -= v.x * x[2 * m2] + v.y * x[2 * m2 + 1];
-                }
-                z[i] = s;
-            }
-#pragma unroll
-            for (int i = 0; i < TRSM_CH; ++i)
-                if (k0 + i < i0) Z[(long long)(k0 + i) * W] = z[i];
-        }
-    }
-}
-
-hipError_t launch_trsm_upper(double* buf, long long stride, int N, int W, int c0, int P, int batch,
-                             hipStream_t s)
-{
-    if (N > QR_MAXN) return hipErrorInvalidValue;
-    dim3 grid((P + 255) / 256, batch);
-    hipLaunchKernelGGL(trsm_kernel, grid, dim3(256), 0, s, buf, stride, N, W, c0, P);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------------------------
-//  Reference-path QR and triangular solve (one column / row at a time through global memory).
-//  These are the kernels the GPU parity suite was first validated with; the engine uses them
-//  unless the blocked kernels above are enabled (DQMC_EXPERIMENTAL_KERNELS=1).
+//  One-kernel Householder QR (DQMC_QR=v1): one 256-thread workgroup per chain factors panels of
+//  QR_NB columns in LDS and updates the trailing columns thread-per-column through global memory.
 // ---------------------------------------------------------------------------------------------
 
 
@@ -1770,8 +1544,10 @@ hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int 
 }
 
 // ---------------------------------------------------------------------------------------------
-//  Upper triangular solve, thread-per-right-hand-side column, 16-row blocks from the bottom.
+//  Upper triangular solve (DQMC_TRSM=v1), thread-per-right-hand-side column, 16-row blocks from
+//  the bottom.
 // ---------------------------------------------------------------------------------------------
+#define TRSM_BR 16
 __global__ __launch_bounds__(256) void trsm_v1_kernel(double* __restrict__ bufs, long long stride, int N, int W,
                                                    int c0, int P)
 {
@@ -1865,7 +1641,8 @@ hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long l
 __global__ __launch_bounds__(256) void udv_finish_kernel(const double* __restrict__ bufs, long long sbuf,
                                                          const int* __restrict__ perms, double* __restrict__ Us,
                                                          long long sU, double* __restrict__ Ds, long long sD,
-                                                         double* __restrict__ Tts, long long sTt, int N)
+                                                         double* __restrict__ Tts, long long sTt, int N,
+                                                         uint32_t* health)
 {
     const int b = blockIdx.y;
     const double* buf = bufs + (long long)b * sbuf;
@@ -1880,16 +1657,21 @@ __global__ __launch_bounds__(256) void udv_finish_kernel(const double* __restric
     // U column-major: U(j, i) = Q(j, i) = (Q^T)(i, j) = buf[i][N + j]
     U[j + (long long)i * N] = buf[(long long)i * W + N + j];
     const double d = fabs(buf[(long long)i * W + i]);
-    if (j == 0) D[i] = d;
+    if (j == 0) {
+        D[i] = d;
+        // a zero or non-finite |R_ii| is a breakdown of the factorization
+        if (!(d > 0.0) || !isfinite(d)) atomicOr(health + b, HEALTH_ZERO_PIVOT);
+    }
     // Tt(perm[j], i) = R(i, j) / D_i for j >= i
     Tt[perm[j] + (long long)i * N] = (j >= i) ? buf[(long long)i * W + j] / d : 0.0;
 }
 
 hipError_t launch_udv_finish(const double* buf, long long sbuf, const int* perm, double* U, long long sU,
-                             double* D, long long sD, double* Tt, long long sTt, int N, int batch, hipStream_t s)
+                             double* D, long long sD, double* Tt, long long sTt, int N, uint32_t* health, int batch,
+                             hipStream_t s)
 {
     dim3 grid((unsigned)(((long long)N * N + 255) / 256), batch);
-    hipLaunchKernelGGL(udv_finish_kernel, grid, dim3(256), 0, s, buf, sbuf, perm, U, sU, D, sD, Tt, sTt, N);
+    hipLaunchKernelGGL(udv_finish_kernel, grid, dim3(256), 0, s, buf, sbuf, perm, U, sU, D, sD, Tt, sTt, N, health);
     return hipGetLastError();
 }
 
@@ -1979,7 +1761,8 @@ __device__ __forceinline__ double block_max(double v, double* red)
 
 __global__ __launch_bounds__(256) void compare_commit_kernel(const double* __restrict__ Gnew, double* __restrict__ G64,
                                                              float* __restrict__ G32, long long sG, int N,
-                                                             double* err_signed, double* err_abs)
+                                                             double* err_signed, double* err_abs,
+                                                             const double* __restrict__ bound, uint32_t* health)
 {
     __shared__ double red[4];
     const int b = blockIdx.x;
@@ -1994,7 +1777,7 @@ __global__ __launch_bounds__(256) void compare_commit_kernel(const double* __res
         const double d = v - cur;
         es = fmax(es, d);
         ea = fmax(ea, fabs(d));
-        bad |= !isfinite(v);
+        bad |= !isfinite(v) || !isfinite(cur);
         g64[e] = v;
         if (g32) g32[e] = (float)v;
     }
@@ -2004,34 +1787,66 @@ __global__ __launch_bounds__(256) void compare_commit_kernel(const double* __res
     if (threadIdx.x == 0) {
         if (es > err_signed[b]) err_signed[b] = es;
         if (ea > err_abs[b] || nanflag > 0.0) err_abs[b] = nanflag > 0.0 ? INFINITY : fmax(err_abs[b], ea);
+        uint32_t h = health[b];
+        if (nanflag > 0.0) h |= HEALTH_NONFINITE_G;
+        const double lim = *bound;
+        if (lim > 0.0 && ea > lim) h |= HEALTH_WATCHDOG;
+        health[b] = h;
     }
 }
 
 hipError_t launch_compare_commit(const double* Gnew, double* G64, float* G32, long long sG, int N,
-                                 double* err_signed, double* err_abs, int batch, hipStream_t s)
+                                 double* err_signed, double* err_abs, const double* bound, uint32_t* health,
+                                 int batch, hipStream_t s)
 {
     hipLaunchKernelGGL(compare_commit_kernel, dim3(batch), dim3(256), 0, s, Gnew, G64, G32, sG, N, err_signed,
-                       err_abs);
+                       err_abs, bound, health);
+    return hipGetLastError();
+}
+
+// Flags chains whose N x N matrix holds a non-finite entry (one workgroup per chain).
+__global__ __launch_bounds__(256) void finite_check_kernel(const double* __restrict__ A, long long sA, int N,
+                                                           uint32_t* health)
+{
+    __shared__ double red[4];
+    const int b = blockIdx.x;
+    int bad = 0;
+    for (long long e = threadIdx.x; e < (long long)N * N; e += 256) bad |= !isfinite(A[(long long)b * sA + e]);
+    const double f = block_max(bad ? 1.0 : 0.0, red);
+    if (threadIdx.x == 0 && f > 0.0) health[b] |= HEALTH_NONFINITE_G;
+}
+
+hipError_t launch_finite_check(const double* A, long long sA, int N, uint32_t* health, int batch, hipStream_t s)
+{
+    hipLaunchKernelGGL(finite_check_kernel, dim3(batch), dim3(256), 0, s, A, sA, N, health);
     return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void compare_only_kernel(const double* __restrict__ Gnew,
                                                            const double* __restrict__ Gcur, long long sG, int N,
-                                                           double* err_signed)
+                                                           double* err_signed, uint32_t* health)
 {
     __shared__ double red[4];
     const int b = blockIdx.x;
     double es = -DBL_MAX;
-    for (long long e = threadIdx.x; e < (long long)N * N; e += 256)
-        es = fmax(es, Gnew[(long long)b * sG + e] - Gcur[(long long)b * sG + e]);
+    int bad = 0;
+    for (long long e = threadIdx.x; e < (long long)N * N; e += 256) {
+        const double v = Gnew[(long long)b * sG + e], cur = Gcur[(long long)b * sG + e];
+        es = fmax(es, v - cur);
+        bad |= !isfinite(v) || !isfinite(cur);
+    }
     es = block_max(es, red);
-    if (threadIdx.x == 0 && es > err_signed[b]) err_signed[b] = es;
+    const double f = block_max(bad ? 1.0 : 0.0, red);
+    if (threadIdx.x == 0) {
+        if (es > err_signed[b]) err_signed[b] = es;
+        if (f > 0.0) health[b] |= HEALTH_NONFINITE_G;
+    }
 }
 
 hipError_t launch_compare_only(const double* Gnew, const double* Gcur, long long sG, int N, double* err_signed,
-                               int batch, hipStream_t s)
+                               uint32_t* health, int batch, hipStream_t s)
 {
-    hipLaunchKernelGGL(compare_only_kernel, dim3(batch), dim3(256), 0, s, Gnew, Gcur, sG, N, err_signed);
+    hipLaunchKernelGGL(compare_only_kernel, dim3(batch), dim3(256), 0, s, Gnew, Gcur, sG, N, err_signed, health);
     return hipGetLastError();
 }
 
@@ -2105,6 +1920,18 @@ hipError_t launch_fill_d(double* dst, long long n, double v, hipStream_t st)
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, n, v);
+    return hipGetLastError();
+}
+
+__global__ void fill_u32_kernel(uint32_t* dst, long long n, uint32_t v)
+{
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e < n) dst[e] = v;
+}
+hipError_t launch_fill_u32(uint32_t* dst, long long n, uint32_t v, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, n, v);
     return hipGetLastError();
 }
 

@@ -16,11 +16,16 @@ LIB_PATH = os.path.join(_HERE, "libdqmc_amd.so")
 
 FP64, MIXED = 0, 1
 GF_CURRENT, GF_TT, GF_T0, GF_0T = 0, 1, 2, 3
+HEALTH_NONFINITE_G, HEALTH_ZERO_PIVOT, HEALTH_RATIO, HEALTH_WATCHDOG = 1, 2, 4, 8
 _ERRS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "ENUMERIC", -5: "ENOMEM", -6: "ENODEV"}
 
 
 class DqmcError(RuntimeError):
-    pass
+    """A failed ABI call; `code` is the DQMC_E* return value."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 class Params(C.Structure):
@@ -80,6 +85,8 @@ def lib():
             "dqmc_forward_fixed": (C.c_int, [vp, dp, dp]),
             "dqmc_adjoint": (C.c_int, [vp, dp, dp, dp]),
             "dqmc_dims": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "dqmc_set_watchdog": (C.c_int, [vp, C.c_double]),
+            "dqmc_get_health": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint32)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -104,11 +111,12 @@ class Engine:
             if h.value:
                 lib().dqmc_destroy(h)
             self._h = C.c_void_p()
-            raise DqmcError("dqmc_create failed: %s %s" % (_ERRS.get(rc, rc), (msg or b"").decode()))
+            raise DqmcError("dqmc_create failed: %s %s" % (_ERRS.get(rc, rc), (msg or b"").decode()), rc)
 
     def _chk(self, rc, what):
         if rc:
-            raise DqmcError("%s failed: %s %s" % (what, _ERRS.get(rc, rc), lib().dqmc_last_error(self._h).decode()))
+            raise DqmcError("%s failed: %s %s" % (what, _ERRS.get(rc, rc), lib().dqmc_last_error(self._h).decode()),
+                            rc)
 
     def close(self):
         if self._h and self._h.value:
@@ -153,6 +161,16 @@ class Engine:
         s = np.zeros((self.nt, self.n), dtype=np.int8)
         self._chk(lib().dqmc_get_fields(self._h, chain, s.ctypes.data_as(C.POINTER(C.c_int8))), "get_fields")
         return s
+
+    def set_watchdog(self, max_dG):
+        """Sweeps return DQMC_ENUMERIC when max|G_recomputed - G_working| exceeds max_dG (0 = off)."""
+        self._chk(lib().dqmc_set_watchdog(self._h, float(max_dG)), "set_watchdog")
+
+    def health(self, chain):
+        """HEALTH_* flags one chain raised during the last call."""
+        x = C.c_uint32()
+        self._chk(lib().dqmc_get_health(self._h, chain, C.byref(x)), "get_health")
+        return int(x.value)
 
     def use_graphs(self, on=True):
         self._chk(lib().dqmc_use_graphs(self._h, int(on)), "use_graphs")

@@ -175,6 +175,27 @@ int dqmc_forward_fixed(dqmc_ctx* ctx, double* logdet, double* gdiag);
  * No counterpart in the reference. */
 int dqmc_adjoint(dqmc_ctx* ctx, const double* gbar_logdet, const double* gbar_gdiag, double* grad);
 
+/* Numeric health (no counterpart in the reference, which throws only on SVD non-convergence,
+ * include/utils/linear_algebra.hpp:77-80, and otherwise lets NaN propagate).  Every dqmc_init,
+ * dqmc_sweep_*, dqmc_forward_fixed and dqmc_adjoint call clears the per-chain flags, runs to
+ * completion, and returns DQMC_ENUMERIC when a chain raised one:
+ *   DQMC_HEALTH_NONFINITE_G  NaN/Inf in the working G or its FP64 recompute at a stabilization
+ *                            (a zero pivot of the recompose solve ends here too)
+ *   DQMC_HEALTH_ZERO_PIVOT   zero or non-finite |R_ii| in a UDV factorization (QR breakdown)
+ *   DQMC_HEALTH_RATIO        non-finite acceptance ratio (non-finite G_ii at a site visit)
+ *   DQMC_HEALTH_WATCHDOG     max_ij |G_recomputed - G_working| at a stabilization above the bound
+ *                            set by dqmc_set_watchdog (0 = off, the default)
+ * After DQMC_ENUMERIC the flagged chain's G is not trustworthy; dqmc_init rebuilds it from the
+ * fields.  dqmc_get_health returns the flags of the last call for one chain. */
+enum {
+    DQMC_HEALTH_NONFINITE_G = 1,
+    DQMC_HEALTH_ZERO_PIVOT = 2,
+    DQMC_HEALTH_RATIO = 4,
+    DQMC_HEALTH_WATCHDOG = 8
+};
+int dqmc_set_watchdog(dqmc_ctx* ctx, double max_dG);
+int dqmc_get_health(dqmc_ctx* ctx, int chain, uint32_t* flags);
+
 /* Lattice size / chain count accessors. */
 int dqmc_dims(dqmc_ctx* ctx, int* ns, int* nt, int* n_chains);
 

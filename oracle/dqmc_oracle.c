@@ -938,6 +938,8 @@ int orc_get_gf(orc_ctx* c, int kind, int t, double* out)
     return -1;
 }
 void orc_set_gftt(orc_ctx* c, const double* g) { mat_copy(c->n, c->gftt, g); }
+/* matrix_compare_error itself, exported for its own tests */
+double orc_compare_error(int n, const double* a, const double* b) { return compare_error(n, a, b); }
 void orc_get_stats(orc_ctx* c, double* wrap_eq, double* wrap_dyn, long long* accepts, long long* visits)
 {
     *wrap_eq = c->wrap_eq; *wrap_dyn = c->wrap_dyn; *accepts = c->accepts; *visits = c->visits;

@@ -62,6 +62,7 @@ def lib():
             "orc_sweep_dynamic": (C.c_int, [vp]),
             "orc_get_gf": (C.c_int, [vp, C.c_int, C.c_int, _dp]),
             "orc_set_gftt": (None, [vp, _dp]),
+            "orc_compare_error": (C.c_double, [C.c_int, _dp, _dp]),
             "orc_get_stats": (None, [vp, _dp, _dp, _llp, _llp]),
             "orc_get_accmask": (None, [vp, _u32p]),
             "orc_current_t": (C.c_int, [vp]),
@@ -131,6 +132,13 @@ def inverse(a):
     out = np.zeros_like(a, order="F")
     rc = lib().orc_inverse(a.shape[0], _ptr(a), _ptr(out))
     return rc, out
+
+
+def compare_error(a, b):
+    """StableNumerics::matrix_compare_error: the signed max of (a - b)."""
+    a = np.asfortranarray(a, dtype=np.float64)
+    b = np.asfortranarray(b, dtype=np.float64)
+    return float(lib().orc_compare_error(a.shape[0], _ptr(a), _ptr(b)))
 
 
 def fullpiv_qr_solve(h, b):

@@ -223,3 +223,48 @@ def test_observables_u0_closed_form():
     want_spin = np.array([np.mean(np.exp(t * eps) * f) * np.mean(np.exp(-t * eps) * (1 - f)) for t in tau])
     assert np.abs(ldos - want_ldos).max() < 1e-11
     assert np.abs(spin - want_spin).max() < 1e-11
+
+
+def test_compare_error_is_signed_max():
+    """StableNumerics::matrix_compare_error (include/utils/stable_numerics.hpp:38-48) is the signed
+    max of (a - b), not the max-abs: a pair whose largest |a - b| entry is negative returns the
+    largest positive difference, and an all-negative difference returns a negative number."""
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal((6, 6))
+    a = b.copy()
+    a[1, 2] -= 1e-3              # largest |difference|, negative
+    a[4, 0] += 2e-4              # largest positive difference
+    assert orc.compare_error(a, b) == (a - b).max()
+    assert abs(orc.compare_error(a, b) - 2e-4) < 1e-15
+    c = b - 0.5
+    assert orc.compare_error(c, b) == (c - b).max() and (c - b).max() < -0.49
+
+
+def _planted_chain(plant):
+    """U = 0, nnt = 0, mu = 0: B = I, G = I/2 exactly and every flip is accepted with gamma = 0,
+    so a matrix planted into G after init reaches the only stabilization of a sweep (nt = pace)
+    unchanged, and the wrap error is exactly the signed max of (I/2 - (I/2 + P)) = max(-P),
+    taken into a running max that starts at 0 (src/dqmc_core.cpp:211-212)."""
+    nl, nt = 4, 4
+    o = orc.Chain(nl, nt, 1.0, 0.0, 0.0, nnt=0.0, pace=nt)
+    o.set_seed(5)
+    o.fields_random()
+    assert o.init() == 0
+    g0 = o.gf(0)
+    assert np.array_equal(g0, 0.5 * np.eye(nl * nl))
+    o.set_gftt(g0 + plant)
+    assert o.sweep_up() == 0
+    assert np.array_equal(o.gf(0), g0)
+    st = o.stats()
+    o.close()
+    return st["wrap_eq"]
+
+
+def test_wrap_error_running_signed_max_planted():
+    n = 16
+    p = np.zeros((n, n))
+    p[1, 2] = 1e-3               # recompute - working = -1e-3: the largest |difference|
+    p[3, 0] = -2e-4              # recompute - working = +2e-4: the signed max
+    assert _planted_chain(p) == 2e-4
+    # every difference negative: the running max stays at its initial 0, neither -eps nor +eps
+    assert _planted_chain(np.full((n, n), 1e-3)) == 0.0
