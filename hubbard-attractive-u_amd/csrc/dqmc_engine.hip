@@ -106,6 +106,12 @@ struct dqmc_ctx {
     double *vecgftt = nullptr, *vecgft0 = nullptr, *vecgf0t = nullptr;
     double *wrap_eq = nullptr, *wrap_dyn = nullptr, *maxdG = nullptr;
     uint32_t* health = nullptr;      // [B] HEALTH_* flags of the current call
+    // observables measured on the device: cache [B][cstride] (equal-time scalars, G(k,t) [nk][L],
+    // LDOS [L], chi_perp [L]), bins [B][nbins][cstride], per-slice partials, cos(k . r_d) [nk][N]
+    unsigned obs_mask = 0;
+    int nbins = 0, nk = 0, cstride = 0;
+    long long eq_count = 0, dyn_count = 0;
+    double *meas_cache = nullptr, *meas_data = nullptr, *meas_part = nullptr, *costab = nullptr;
     double* wd_bound = nullptr;      // watchdog bound on max|G_recomputed - G_working| (0 = off)
     DStack left, right;
     // parameter adjoint at fixed fields (allocated on first use)
@@ -149,6 +155,18 @@ struct dqmc_ctx {
         int _r = (expr);            \
         if (_r) return _r;          \
     } while (0)
+
+namespace dqmc {
+namespace {
+thread_local std::string g_free_err;
+}
+int free_error(int code, const char* msg)
+{
+    g_free_err = msg;
+    return code;
+}
+const char* free_error_text() { return g_free_err.c_str(); }
+}  // namespace dqmc
 
 namespace {
 
@@ -200,27 +218,6 @@ void lattice_expk(int nl, double x, std::vector<double>& out)
         for (int i = 0; i < N; ++i) {
             const int dx = ((j % nl) - (i % nl) + nl) % nl, dy = ((j / nl) - (i / nl) + nl) % nl;
             out[i + (size_t)j * N] = f[dx + nl * dy];
-        }
-}
-
-// exp(x A1) for the periodic ring of nl sites (A1[i][i+1] += 1, A1[i+1][i] += 1, so nl = 2 gives
-// 2 and nl = 1 gives 2 on the diagonal), padded to a 16 x 16 row-major block.  The square
-// lattice's exp(x A) is its Kronecker square.
-void ring_expk16(int nl, double x, std::vector<double>& out)
-{
-    out.assign(256, 0.0);
-    if (x == 0.0) {   // exactly the identity
-        for (int i = 0; i < nl; ++i) out[i * 16 + i] = 1.0;
-        return;
-    }
-    for (int i = 0; i < nl; ++i)
-        for (int j = 0; j < nl; ++j) {
-            double s = 0.0;
-            for (int m = 0; m < nl; ++m) {
-                const double k = 2.0 * M_PI * m / nl;
-                s += std::cos(k * (i - j)) * std::exp(2.0 * x * std::cos(k));
-            }
-            out[i * 16 + j] = s / nl;
         }
 }
 
@@ -783,18 +780,21 @@ extern "C" {
 
 int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k, dqmc_ctx** out)
 {
-    if (!p || !out || n_chains <= 0) return DQMC_EINVAL;
+    using dqmc::free_error;
+    if (!p || !out || n_chains <= 0) return free_error(DQMC_EINVAL, "dqmc_create: null argument or n_chains < 1");
     *out = nullptr;
-    if (p->nl < 1 || p->nt < 1 || p->stabilization_pace < 1 || p->u < 0.0 || p->beta < 0.0) return DQMC_EINVAL;
-    if (precision != DQMC_FP64 && precision != DQMC_MIXED) return DQMC_EINVAL;
+    if (p->nl < 1 || p->nt < 1 || p->stabilization_pace < 1 || p->u < 0.0 || p->beta < 0.0)
+        return free_error(DQMC_EINVAL, "dqmc_create: need nl >= 1, nt >= 1, stabilization_pace >= 1, u >= 0, beta >= 0");
+    if (precision != DQMC_FP64 && precision != DQMC_MIXED) return free_error(DQMC_EINVAL, "dqmc_create: bad precision");
     const int N = p->nl * p->nl;
-    if (p->nl > 16) return DQMC_EINVAL;   // N <= 256: Kronecker tiles and QR panels
-    if (delay_k < 1 || delay_k > 32) return DQMC_EINVAL;
+    if (p->nl > 16) return free_error(DQMC_EINVAL, "dqmc_create: nl > 16 is not supported (N <= 256)");
+    if (delay_k < 1 || delay_k > 32) return free_error(DQMC_EINVAL, "dqmc_create: delay_k must be 1..32");
     const size_t esz = precision == DQMC_MIXED ? 4 : 8;
-    if (metropolis_lds_bytes(N, delay_k, esz) > 160 * 1024) return DQMC_EINVAL;
+    if (metropolis_lds_bytes(N, delay_k, esz) > 160 * 1024)
+        return free_error(DQMC_EINVAL, "dqmc_create: the Metropolis block does not fit in LDS");
 
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return DQMC_ENODEV;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return free_error(DQMC_ENODEV, "dqmc_create: no GPU");
 
     dqmc_ctx* c = new dqmc_ctx();
     c->nl = p->nl; c->N = N; c->L = p->nt; c->pace = p->stabilization_pace;
@@ -814,7 +814,7 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
             // the pipelined flush needs waves beside the site-loop wave: 256, 512 or 1024 threads
             if (v != 256 && v != 512 && v != 1024) {
                 delete c;
-                return DQMC_EINVAL;
+                return free_error(DQMC_EINVAL, "dqmc_create: DQMC_METRO_THREADS must be 256, 512 or 1024");
             }
             c->metro_threads = v;
         }
@@ -882,9 +882,9 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         HCHECK(c, hipMemcpy(c->invExpK32, f.data(), nn * 4, hipMemcpyHostToDevice));
     }
     {
-        std::vector<double> e1, f1;
-        ring_expk16(c->nl, c->dt * c->nnt, e1);
-        ring_expk16(c->nl, -c->dt * c->nnt, f1);
+        std::vector<double> e1(256), f1(256);
+        ring_expk16(c->nl, c->dt * c->nnt, e1.data());
+        ring_expk16(c->nl, -c->dt * c->nnt, f1.data());
         std::vector<float> e1f(256), f1f(256);
         for (int i = 0; i < 256; ++i) { e1f[i] = (float)e1[i]; f1f[i] = (float)f1[i]; }
         HCHECK(c, hipMemcpy(c->E1, e1.data(), 256 * 8, hipMemcpyHostToDevice));
@@ -921,7 +921,7 @@ void dqmc_destroy(dqmc_ctx* c)
     delete c;
 }
 
-const char* dqmc_last_error(dqmc_ctx* c) { return c ? c->err.c_str() : "null context"; }
+const char* dqmc_last_error(dqmc_ctx* c) { return c ? c->err.c_str() : dqmc::free_error_text(); }
 
 static int check_chain(dqmc_ctx* c, int chain)
 {
@@ -1051,6 +1051,60 @@ int dqmc_apply_bop(dqmc_ctx* c, int chain, int op, int t, double* inout)
     }
     HCHECK(c, hipStreamSynchronize(c->stream));
     HCHECK(c, hipMemcpy(inout, c->W3 + chain * nn, nn * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ---- single-site operations of Model::HubbardAttractiveU (include/hubbard.h:71-73) ----------
+static int site_args(dqmc_ctx* c, int chain, int t, int i, const char* what)
+{
+    RCHECK(check_chain(c, chain));
+    if (t < 0 || t >= c->L || i < 0 || i >= c->N) return c->fail(DQMC_EINVAL, "%s: (t, i) out of range", what);
+    if (!c->initialized) return c->fail(DQMC_ESTATE, "%s before dqmc_init", what);
+    if (c->prec != DQMC_FP64) return c->fail(DQMC_EINVAL, "%s needs an FP64 context", what);
+    return 0;
+}
+static int site_delta(dqmc_ctx* c, int chain, int t, int i, double* delta)
+{
+    int8_t s = 0;
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(&s, c->fields + ((size_t)chain * c->L + t) * c->N + i, 1, hipMemcpyDeviceToHost));
+    *delta = std::exp(-2 * c->alpha * s) - 1;   // src/hubbard.cpp:127 (the table of dqmc_init)
+    return 0;
+}
+
+int dqmc_acceptance_ratio(dqmc_ctx* c, int chain, int t, int i, double* ratio)
+{
+#pragma clang fp contract(off)
+    RCHECK(site_args(c, chain, t, i, "dqmc_acceptance_ratio"));
+    if (!ratio) return DQMC_EINVAL;
+    double delta = 0.0, gii = 0.0;
+    RCHECK(site_delta(c, chain, t, i, &delta));
+    HCHECK(c, hipMemcpy(&gii, c->G64 + (size_t)chain * c->NN() + (size_t)i * c->N + i, 8, hipMemcpyDeviceToHost));
+    const double x = 1 + (1 - gii) * delta;   // src/hubbard.cpp:123-130
+    *ratio = x * x / (1 + delta);
+    return 0;
+}
+
+int dqmc_update_green_function(dqmc_ctx* c, int chain, int t, int i)
+{
+    RCHECK(site_args(c, chain, t, i, "dqmc_update_green_function"));
+    double delta = 0.0;
+    RCHECK(site_delta(c, chain, t, i, &delta));
+    HCHECK(c, launch_site_update(c->G64 + (size_t)chain * c->NN(), c->N, i, delta, c->stream));
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int dqmc_flip_field(dqmc_ctx* c, int chain, int t, int i)
+{
+    RCHECK(check_chain(c, chain));
+    if (t < 0 || t >= c->L || i < 0 || i >= c->N) return c->fail(DQMC_EINVAL, "dqmc_flip_field: (t, i) out of range");
+    int8_t s = 0;
+    int8_t* p = c->fields + ((size_t)chain * c->L + t) * c->N + i;
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(&s, p, 1, hipMemcpyDeviceToHost));
+    s = (int8_t)-s;
+    HCHECK(c, hipMemcpy(p, &s, 1, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -1322,6 +1376,102 @@ int dqmc_get_health(dqmc_ctx* c, int chain, uint32_t* flags)
     if (!flags) return DQMC_EINVAL;
     HCHECK(c, hipStreamSynchronize(c->stream));
     HCHECK(c, hipMemcpy(flags, c->health + chain, 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int dqmc_measure_setup(dqmc_ctx* c, unsigned obs_mask, int nbins, int nk, const double* kxy)
+{
+    if (!c) return DQMC_EINVAL;
+    if (obs_mask == 0 || (obs_mask & ~63u) || nbins < 1 || nk < 0 || (nk > 0 && !kxy))
+        return c->fail(DQMC_EINVAL, "dqmc_measure_setup: bad observable mask, bin count or momentum list");
+    if ((obs_mask & DQMC_OBS_GREEN_KT) && nk < 1)
+        return c->fail(DQMC_EINVAL, "dqmc_measure_setup: DynamicGreenFunctions needs a momentum list");
+    if ((obs_mask & 7u) && !c->eqtime)
+        return c->fail(DQMC_ESTATE, "dqmc_measure_setup: equal-time observables need dqmc_set_collect(eqtime)");
+    if ((obs_mask & 56u) && !c->dynamic)
+        return c->fail(DQMC_ESTATE, "dqmc_measure_setup: dynamic observables need dqmc_set_collect(dynamic)");
+    if (c->meas_cache) return c->fail(DQMC_ESTATE, "dqmc_measure_setup: already set up");
+    const int B = c->B, L = c->L, N = c->N;
+    c->obs_mask = obs_mask; c->nbins = nbins; c->nk = nk;
+    c->cstride = 3 + nk * L + 2 * L;
+    RCHECK(c->alloc(&c->meas_cache, (size_t)B * c->cstride * 8));
+    RCHECK(c->alloc(&c->meas_data, (size_t)B * nbins * c->cstride * 8));
+    RCHECK(c->alloc(&c->meas_part, (size_t)B * L * 3 * 8));
+    RCHECK(c->alloc(&c->costab, (size_t)std::max(1, nk) * N * 8));
+    // Re exp(-i k . r_d) for the displacement of site index d (src/square_lattice.cpp:120-135)
+    std::vector<double> ct((size_t)std::max(1, nk) * N, 0.0);
+    for (int k = 0; k < nk; ++k)
+        for (int d = 0; d < N; ++d) ct[(size_t)k * N + d] = std::cos((d % c->nl) * kxy[2 * k] + (d / c->nl) * kxy[2 * k + 1]);
+    HCHECK(c, hipMemcpy(c->costab, ct.data(), ct.size() * 8, hipMemcpyHostToDevice));
+    HCHECK(c, launch_fill_d(c->meas_cache, (long long)B * c->cstride, 0.0, c->stream));
+    HCHECK(c, launch_fill_d(c->meas_data, (long long)B * nbins * c->cstride, 0.0, c->stream));
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    c->eq_count = c->dyn_count = 0;
+    return 0;
+}
+
+int dqmc_measure_equaltime(dqmc_ctx* c)
+{
+    if (!c) return DQMC_EINVAL;
+    if (!c->meas_cache || !(c->obs_mask & 7u)) return c->fail(DQMC_ESTATE, "dqmc_measure_equaltime: no equal-time observables set up");
+    if (!c->initialized) return c->fail(DQMC_ESTATE, "dqmc_measure_equaltime before dqmc_init");
+    HCHECK(c, launch_measure_equaltime(c->vecgftt, (long long)c->L * c->NN(), c->L, c->N, c->meas_part, c->meas_cache,
+                                       c->cstride, c->B, c->stream));
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    c->eq_count += 1;
+    return 0;
+}
+
+int dqmc_measure_dynamic(dqmc_ctx* c)
+{
+    if (!c) return DQMC_EINVAL;
+    if (!c->meas_cache || !(c->obs_mask & 56u)) return c->fail(DQMC_ESTATE, "dqmc_measure_dynamic: no dynamic observables set up");
+    if (!c->initialized) return c->fail(DQMC_ESTATE, "dqmc_measure_dynamic before dqmc_init");
+    const int flags = ((c->obs_mask & DQMC_OBS_GREEN_KT) ? 1 : 0) | ((c->obs_mask & DQMC_OBS_LDOS) ? 2 : 0) |
+                      ((c->obs_mask & DQMC_OBS_SPIN) ? 4 : 0);
+    HCHECK(c, launch_measure_dynamic(c->vecgft0, c->vecgf0t, (long long)c->L * c->NN(), c->L, c->nl, c->costab, c->nk,
+                                     c->meas_cache, c->cstride, flags, c->B, c->stream));
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    c->dyn_count += 1;
+    return 0;
+}
+
+int dqmc_measure_push_bin(dqmc_ctx* c, int bin)
+{
+    if (!c) return DQMC_EINVAL;
+    if (!c->meas_cache) return c->fail(DQMC_ESTATE, "dqmc_measure_push_bin: no observables set up");
+    if (bin < 0 || bin >= c->nbins)
+        return c->fail(DQMC_EINVAL, "Observable::Observable<_scalar>::push_cache_to_data(): index of bin out of range.");
+    if (((c->obs_mask & 7u) && c->eq_count == 0) || ((c->obs_mask & 56u) && c->dyn_count == 0))
+        return c->fail(DQMC_ESTATE, "Observable::Observable<_scalar>::normalize_cache(): cache divided by zero.");
+    HCHECK(c, launch_measure_push(c->meas_cache, c->cstride, c->meas_data, (long long)c->nbins * c->cstride, bin, 3,
+                                  c->eq_count, c->dyn_count, c->B, c->stream));
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    c->eq_count = c->dyn_count = 0;
+    return 0;
+}
+
+int dqmc_get_bins(dqmc_ctx* c, int chain, unsigned obs, double* out)
+{
+    RCHECK(check_chain(c, chain));
+    if (!out || !c->meas_cache) return c->fail(DQMC_EINVAL, "dqmc_get_bins: no output or no observables set up");
+    if (!(obs & c->obs_mask) || (obs & (obs - 1))) return c->fail(DQMC_EINVAL, "dqmc_get_bins: observable %u not measured", obs);
+    const int L = c->L;
+    std::vector<double> d((size_t)c->nbins * c->cstride);
+    HCHECK(c, hipStreamSynchronize(c->stream));
+    HCHECK(c, hipMemcpy(d.data(), c->meas_data + (size_t)chain * c->nbins * c->cstride, d.size() * 8,
+                        hipMemcpyDeviceToHost));
+    int off = 0, len = 1;
+    switch (obs) {
+        case DQMC_OBS_FILLING: off = 0; break;
+        case DQMC_OBS_DOUBLE_OCCU: off = 1; break;
+        case DQMC_OBS_SWAVE: off = 2; break;
+        case DQMC_OBS_GREEN_KT: off = 3; len = c->nk * L; break;
+        case DQMC_OBS_LDOS: off = 3 + c->nk * L; len = L; break;
+        default: off = 3 + c->nk * L + L; len = L; break;
+    }
+    for (int b = 0; b < c->nbins; ++b)
+        std::memcpy(out + (size_t)b * len, d.data() + (size_t)b * c->cstride + off, (size_t)len * 8);
     return 0;
 }
 

@@ -63,6 +63,13 @@ struct KronArgs {
     DiagArg rowd, innd, cold;
     int batch;
 };
+// Host: the message of the calling thread's last failed call without a context
+// (dqmc_last_error(NULL)); free_error records it and returns code.
+int free_error(int code, const char* msg);
+const char* free_error_text();
+
+// Host: the 16 x 16 padded factor E = exp(x A1) of the ring of nl sites (x = 0: the identity).
+void ring_expk16(int nl, double x, double* out);
 template <typename T>
 hipError_t launch_kron_left(const KronArgs<T>& a, hipStream_t s);
 template <typename T>
@@ -159,6 +166,10 @@ hipError_t launch_compare_only(const double* Gnew, const double* Gcur, long long
 // HEALTH_NONFINITE_G for chains whose N x N matrix holds a non-finite entry.
 hipError_t launch_finite_check(const double* A, long long sA, int N, uint32_t* health, int batch, hipStream_t s);
 
+// Rank-1 Sherman-Morrison update of one chain's G (N x N column-major) at site i for the field
+// flip with delta = e^{-2 alpha s} - 1 (src/hubbard.cpp:132-139).
+hipError_t launch_site_update(double* G, int N, int i, double delta, hipStream_t s);
+
 // Elementwise helpers (n elements per chain).
 hipError_t launch_set_identity(double* A, long long s, int N, double diagval, int batch, hipStream_t st);
 hipError_t launch_copy_d(const double* src, long long ss, double* dst, long long sd, long long n, int batch,
@@ -172,6 +183,20 @@ hipError_t launch_axpi(const double* src, long long ss, double* dst, long long s
                        int batch, hipStream_t st);
 hipError_t launch_fill_d(double* dst, long long n, double v, hipStream_t st);
 hipError_t launch_fill_u32(uint32_t* dst, long long n, uint32_t v, hipStream_t st);
+
+// ---- observables on the device (dqmc_measure.hip) ------------------------------------------
+// Equal-time estimators of all L stored slices folded into cache[b][0..2] (filling, double
+// occupancy, s-wave pairing); part is [batch][L][3] scratch.
+hipError_t launch_measure_equaltime(const double* vec, long long sv, int L, int N, double* part, double* cache,
+                                    int cstride, int batch, hipStream_t s);
+// Dynamic estimators (flags: 1 G(k,t), 2 LDOS, 4 chi_perp) of all L slices added to
+// cache[b][3 ..]; costab [nk][N] holds cos(k . r_d).
+hipError_t launch_measure_dynamic(const double* gt0, const double* g0t, long long sv, int L, int nl,
+                                  const double* costab, int nk, double* cache, int cstride, int flags, int batch,
+                                  hipStream_t s);
+// data[b][bin] = cache[b] / count (eq_count for the first neq entries, dyn_count after), cache = 0.
+hipError_t launch_measure_push(double* cache, int cstride, double* data, long long dstride, int bin, int neq,
+                               long long eq_count, long long dyn_count, int batch, hipStream_t s);
 
 // ---- parameter adjoint at fixed fields (dqmc_adjoint.hip) ----------------------------------
 struct AdjointArgs {

@@ -14,6 +14,8 @@
 
 #include <float.h>
 
+#include <cmath>
+
 namespace dqmc {
 
 // ---------------------------------------------------------------------------------------------
@@ -536,6 +538,27 @@ hipError_t launch_kron_right(const KronArgs<T>& a, hipStream_t s)
         hipLaunchKernelGGL(kron_right_kernel<T>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
+// exp(x A1) for the periodic ring of nl sites (A1[i][i+1] += 1, A1[i+1][i] += 1, so nl = 2 gives
+// 2 and nl = 1 gives 2 on the diagonal), padded to a 16 x 16 row-major block.  The square
+// lattice's exp(x A) is its Kronecker square.  Host code.
+void ring_expk16(int nl, double x, double* out)
+{
+    for (int e = 0; e < 256; ++e) out[e] = 0.0;
+    if (x == 0.0) {   // exactly the identity (the cosine sums below round to ~1e-17)
+        for (int i = 0; i < nl; ++i) out[i * 16 + i] = 1.0;
+        return;
+    }
+    for (int i = 0; i < nl; ++i)
+        for (int j = 0; j < nl; ++j) {
+            double s = 0.0;
+            for (int m = 0; m < nl; ++m) {
+                const double k = 2.0 * M_PI * m / nl;
+                s += std::cos(k * (i - j)) * std::exp(2.0 * x * std::cos(k));
+            }
+            out[i * 16 + j] = s / nl;
+        }
+}
+
 template hipError_t launch_kron_left<double>(const KronArgs<double>&, hipStream_t);
 template hipError_t launch_kron_right<double>(const KronArgs<double>&, hipStream_t);
 template hipError_t launch_kron_left<float>(const KronArgs<float>&, hipStream_t);
@@ -1847,6 +1870,35 @@ hipError_t launch_compare_only(const double* Gnew, const double* Gcur, long long
                                uint32_t* health, int batch, hipStream_t s)
 {
     hipLaunchKernelGGL(compare_only_kernel, dim3(batch), dim3(256), 0, s, Gnew, Gcur, sG, N, err_signed, health);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------------------
+//  Single-site Sherman-Morrison update (Model::HubbardAttractiveU::update_green_function,
+//  src/hubbard.cpp:132-139): G(r, c) -= (gamma G(r, i)) (delta_ic - G(i, c)) with
+//  gamma = delta / (1 + (1 - G_ii) delta) and delta = e^{-2 alpha s(t,i)} - 1, all from the G before
+//  the update.  One workgroup per chain stages column i and row i in LDS first.
+// ---------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void site_update_kernel(double* __restrict__ G, int N, int i, double delta)
+{
+#pragma clang fp contract(off)
+    __shared__ double col[1024], row[1024];
+    for (int r = threadIdx.x; r < N; r += 256) {
+        col[r] = G[r + (long long)i * N];
+        row[r] = G[i + (long long)r * N];
+    }
+    __syncthreads();
+    const double gamma = delta / (1.0 + (1.0 - col[i]) * delta);
+    for (long long e = threadIdx.x; e < (long long)N * N; e += 256) {
+        const int r = (int)(e % N), c = (int)(e / N);
+        const double t = (gamma * col[r]) * ((c == i ? 1.0 : 0.0) - row[c]);
+        G[e] = G[e] - t;
+    }
+}
+hipError_t launch_site_update(double* G, int N, int i, double delta, hipStream_t s)
+{
+    if (N > 1024 || i < 0 || i >= N) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(site_update_kernel, dim3(1), dim3(256), 0, s, G, N, i, delta);
     return hipGetLastError();
 }
 

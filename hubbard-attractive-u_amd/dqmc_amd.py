@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(_HERE, "libdqmc_amd.so")
 FP64, MIXED = 0, 1
 GF_CURRENT, GF_TT, GF_T0, GF_0T = 0, 1, 2, 3
 HEALTH_NONFINITE_G, HEALTH_ZERO_PIVOT, HEALTH_RATIO, HEALTH_WATCHDOG = 1, 2, 4, 8
+OBS_FILLING, OBS_DOUBLE_OCCU, OBS_SWAVE, OBS_GREEN_KT, OBS_LDOS, OBS_SPIN = 1, 2, 4, 8, 16, 32
 _ERRS = {-1: "EINVAL", -2: "EHIP", -3: "ESTATE", -4: "ENUMERIC", -5: "ENOMEM", -6: "ENODEV"}
 
 
@@ -85,6 +86,14 @@ def lib():
             "dqmc_forward_fixed": (C.c_int, [vp, dp, dp]),
             "dqmc_adjoint": (C.c_int, [vp, dp, dp, dp]),
             "dqmc_dims": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "dqmc_measure_setup": (C.c_int, [vp, C.c_uint, C.c_int, C.c_int, dp]),
+            "dqmc_measure_equaltime": (C.c_int, [vp]),
+            "dqmc_measure_dynamic": (C.c_int, [vp]),
+            "dqmc_measure_push_bin": (C.c_int, [vp, C.c_int]),
+            "dqmc_get_bins": (C.c_int, [vp, C.c_int, C.c_uint, dp]),
+            "dqmc_acceptance_ratio": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
+            "dqmc_update_green_function": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+            "dqmc_flip_field": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
             "dqmc_set_watchdog": (C.c_int, [vp, C.c_double]),
             "dqmc_get_health": (C.c_int, [vp, C.c_int, C.POINTER(C.c_uint32)]),
         }
@@ -107,7 +116,7 @@ class Engine:
         rc = lib().dqmc_create(C.byref(p), n_chains, precision, delay_k, C.byref(h))
         self._h = h
         if rc:
-            msg = lib().dqmc_last_error(h) if h.value else b""
+            msg = lib().dqmc_last_error(h if h.value else None)
             if h.value:
                 lib().dqmc_destroy(h)
             self._h = C.c_void_p()
@@ -162,6 +171,32 @@ class Engine:
         self._chk(lib().dqmc_get_fields(self._h, chain, s.ctypes.data_as(C.POINTER(C.c_int8))), "get_fields")
         return s
 
+    # --- observables on the device ---------------------------------------------------------------
+    def measure_setup(self, obs_mask, nbins, kxy=None):
+        """Select the DQMC_OBS_* observables, the bin count and the momentum list [(kx, ky), ...]."""
+        k = np.ascontiguousarray(np.zeros((0, 2)) if kxy is None else kxy, dtype=np.float64).reshape(-1, 2)
+        self._nk, self._nbins = k.shape[0], nbins
+        self._chk(lib().dqmc_measure_setup(self._h, obs_mask, nbins, k.shape[0],
+                                           k.ctypes.data_as(C.POINTER(C.c_double)) if k.size else None),
+                  "measure_setup")
+
+    def measure_equaltime(self):
+        self._chk(lib().dqmc_measure_equaltime(self._h), "measure_equaltime")
+
+    def measure_dynamic(self):
+        self._chk(lib().dqmc_measure_dynamic(self._h), "measure_dynamic")
+
+    def push_bin(self, b):
+        self._chk(lib().dqmc_measure_push_bin(self._h, b), "measure_push_bin")
+
+    def bins(self, chain, obs):
+        """One chain's bins of one observable: [nbins], [nbins, nk, nt] or [nbins, nt]."""
+        shape = {OBS_GREEN_KT: (self._nbins, self._nk, self.nt), OBS_LDOS: (self._nbins, self.nt),
+                 OBS_SPIN: (self._nbins, self.nt)}.get(obs, (self._nbins,))
+        out = np.zeros(shape)
+        self._chk(lib().dqmc_get_bins(self._h, chain, obs, out.ctypes.data_as(C.POINTER(C.c_double))), "get_bins")
+        return out
+
     def set_watchdog(self, max_dG):
         """Sweeps return DQMC_ENUMERIC when max|G_recomputed - G_working| exceeds max_dG (0 = off)."""
         self._chk(lib().dqmc_set_watchdog(self._h, float(max_dG)), "set_watchdog")
@@ -190,6 +225,17 @@ class Engine:
 
     def sweep_dynamic(self):
         self._chk(lib().dqmc_sweep_dynamic(self._h), "sweep_dynamic")
+
+    def acceptance_ratio(self, chain, t, i):
+        r = C.c_double()
+        self._chk(lib().dqmc_acceptance_ratio(self._h, chain, t, i, C.byref(r)), "acceptance_ratio")
+        return r.value
+
+    def update_green_function(self, chain, t, i):
+        self._chk(lib().dqmc_update_green_function(self._h, chain, t, i), "update_green_function")
+
+    def flip_field(self, chain, t, i):
+        self._chk(lib().dqmc_flip_field(self._h, chain, t, i), "flip_field")
 
     def apply_bop(self, chain, op, t, g):
         g = np.array(g, dtype=np.float64, order="F", copy=True)

@@ -19,7 +19,7 @@ namespace DQMC {
 inline void check(dqmc_ctx* c, int rc, const char* what)
 {
     if (rc != DQMC_OK)
-        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + (c ? dqmc_last_error(c) : ""));
+        throw std::runtime_error(std::string(what) + " failed (" + std::to_string(rc) + "): " + dqmc_last_error(c));
 }
 
 // Column-major N x N host matrix: the reference's Eigen::MatrixXd for G accessors.
@@ -81,6 +81,22 @@ class HubbardAttractiveU {
     void multiply_invB_from_left(DQMC::Matrix& g, int t, int chain = 0) const { bop(g, 2, t, chain); }
     void multiply_invB_from_right(DQMC::Matrix& g, int t, int chain = 0) const { bop(g, 3, t, chain); }
     void multiply_adjB_from_left(DQMC::Matrix& g, int t, int chain = 0) const { bop(g, 4, t, chain); }
+    // locally_update_ising_field / get_acceptance_ratio / update_green_function (include/hubbard.h:71-73)
+    // on the chain's current G; the Core argument of the reference is the engine context itself
+    void locally_update_ising_field(int t, int i, int chain = 0)
+    {
+        DQMC::check(ctx, dqmc_flip_field(ctx, chain, t, i), "dqmc_flip_field");
+    }
+    double get_acceptance_ratio(int t, int i, int chain = 0) const
+    {
+        double r = 0.0;
+        DQMC::check(ctx, dqmc_acceptance_ratio(ctx, chain, t, i, &r), "dqmc_acceptance_ratio");
+        return r;
+    }
+    void update_green_function(int t, int i, int chain = 0)
+    {
+        DQMC::check(ctx, dqmc_update_green_function(ctx, chain, t, i), "dqmc_update_green_function");
+    }
 
    private:
     void bop(DQMC::Matrix& g, int op, int t, int chain) const
@@ -128,12 +144,37 @@ class Core {
     double equaltimeWrapError() const { return stat(0); }
     double dynamicWrapError() const { return stat(1); }
 
-    // host copies of all slices of one kind for the measurement kernels
-    std::vector<std::vector<double>> all(int kind, int chain) const
+    // Observables on the device (Measurement::Handle::initialize and the Observable::Methods
+    // estimators, src/measure_handle.cpp:18-104): only the bins ever reach the host.
+    void setup_measurement(const Measurement::Handle& m, const Lattice::SquareLattice& lat)
     {
-        std::vector<std::vector<double>> v(nt, std::vector<double>((size_t)ns * ns));
-        for (int t = 0; t < nt; ++t) check(model->ctx, dqmc_get_gf(model->ctx, chain, kind, t, v[t].data()), "dqmc_get_gf");
-        return v;
+        unsigned mask = 0;
+        for (const auto& n : m.eq_list) mask |= obs_bit(n);
+        for (const auto& n : m.dyn_list) mask |= obs_bit(n);
+        if (!mask || (!m.equaltime && !m.dynamic)) return;
+        std::vector<double> kxy;
+        for (int k : m.momentum_list) { kxy.push_back(lat.kx(k)); kxy.push_back(lat.ky(k)); }
+        check(model->ctx, dqmc_measure_setup(model->ctx, mask, m.bin_num, (int)m.momentum_list.size(), kxy.data()),
+              "dqmc_measure_setup");
+    }
+    void measure_equaltime() { check(model->ctx, dqmc_measure_equaltime(model->ctx), "dqmc_measure_equaltime"); }
+    void measure_dynamic() { check(model->ctx, dqmc_measure_dynamic(model->ctx), "dqmc_measure_dynamic"); }
+    void push_bin(int bin) { check(model->ctx, dqmc_measure_push_bin(model->ctx, bin), "dqmc_measure_push_bin"); }
+    // every chain's bins into its handle's observables (one handle per chain, like one per rank)
+    void fetch_bins(std::vector<Measurement::Handle>& per_chain) const
+    {
+        for (int c = 0; c < chains; ++c)
+            for (auto& kv : per_chain.at(c).obs)
+                check(model->ctx, dqmc_get_bins(model->ctx, c, obs_bit(kv.first), kv.second.data.data()), "dqmc_get_bins");
+    }
+    static unsigned obs_bit(const std::string& n)
+    {
+        if (n == "FillingNumber") return DQMC_OBS_FILLING;
+        if (n == "DoubleOccupation") return DQMC_OBS_DOUBLE_OCCU;
+        if (n == "StaticSWavePairingCorrelation") return DQMC_OBS_SWAVE;
+        if (n == "DynamicGreenFunctions") return DQMC_OBS_GREEN_KT;
+        if (n == "LocalDensityOfStates") return DQMC_OBS_LDOS;
+        return DQMC_OBS_SPIN;
     }
 
    private:
@@ -189,18 +230,18 @@ class Handle {
             for (int s = 1; s <= binsize; ++s) {
                 if (m.dynamic) {
                     core.sweep_for_dynamic_green_functions(model);
-                    for (int c = 0; c < core.chains; ++c)
-                        per_chain[c].measure_dynamic(core.all(DQMC_GF_T0, c), core.all(DQMC_GF_0T, c), core.ns, lat);
+                    core.measure_dynamic();
                 } else {
                     core.sweep_from_0_to_beta(model);
-                    for (int c = 0; c < core.chains; ++c) per_chain[c].measure_equaltime(core.all(DQMC_GF_TT, c), core.ns);
+                    core.measure_equaltime();
                 }
                 core.sweep_from_beta_to_0(model);
-                if (m.equaltime)
-                    for (int c = 0; c < core.chains; ++c) per_chain[c].measure_equaltime(core.all(DQMC_GF_TT, c), core.ns);
+                if (m.equaltime) core.measure_equaltime();
             }
-            for (auto& h : per_chain) { h.normalize_cache(); h.push_cache_to_data(bin); h.clear_cache(); }
+            core.push_bin(bin);   // normalize_cache + push_cache_to_data + clear_cache, every chain
         }
+        core.fetch_bins(per_chain);
+        (void)lat;
     }
 };
 

@@ -471,9 +471,8 @@ inline void print_imaginary_time_grids(std::ostream& os, const Params& p)
 
 namespace Measurement {
 
-// Observable registry, flags and per-measurement kernels (src/measure_handle.cpp,
-// include/observable_methods.hpp).  Measurements read host copies of the Green's functions:
-// gftt[t], gft0[t], gf0t[t], each N x N column-major.
+// Observable registry, flags and bins of one chain (src/measure_handle.cpp).  The estimators of
+// include/observable_methods.hpp run on the device (dqmc_measure_*); the bins land in `obs`.
 class Handle {
    public:
     int sweeps_warmup = 0, bin_num = 0, bin_capacity = 0, sweeps_between_bins = 0;
@@ -513,72 +512,6 @@ class Handle {
             obs[n].name = n;
             if (n == "DynamicGreenFunctions") obs[n].set_shape(nb, {nk, nt});
             else obs[n].set_shape(nb, {nt});
-        }
-    }
-
-    // observable_methods.hpp:39-147 (equal-time; both spins share G)
-    void measure_equaltime(const std::vector<std::vector<double>>& gftt, int ns)
-    {
-        const int nt = (int)gftt.size();
-        for (const auto& n : eq_list) {
-            auto& o = obs[n];
-            double acc = 0.0;
-            for (int t = 0; t < nt; ++t) {
-                const double* g = gftt[t].data();
-                if (n == "FillingNumber") {
-                    double tr = 0.0;
-                    for (int i = 0; i < ns; ++i) tr += g[i + (size_t)i * ns];
-                    acc += 2. - (tr + tr) / ns;
-                } else if (n == "DoubleOccupation") {
-                    for (int i = 0; i < ns; ++i) { const double d = 1. - g[i + (size_t)i * ns]; acc += d * d; }
-                } else {
-                    for (int i = 0; i < ns; ++i)
-                        for (int j = 0; j < ns; ++j) {
-                            const double gc = (i == j ? 1.0 : 0.0) - g[j + (size_t)i * ns];
-                            const double gij = g[i + (size_t)j * ns];
-                            acc += gc * gc + gij * gij;
-                        }
-                }
-            }
-            if (n == "FillingNumber") o.cache[0] += acc / nt;
-            else o.cache[0] += acc / ((double)ns * nt);
-            o.count++;
-        }
-    }
-
-    // observable_methods.hpp:84-195 (dynamic)
-    void measure_dynamic(const std::vector<std::vector<double>>& gft0, const std::vector<std::vector<double>>& gf0t,
-                         int ns, const Lattice::SquareLattice& lat)
-    {
-        const int nt = (int)gft0.size();
-        const int nk = (int)momentum_list.size();
-        for (const auto& n : dyn_list) {
-            auto& o = obs[n];
-            for (int t = 0; t < nt; ++t) {
-                const double* g = gft0[t].data();
-                const double* h = gf0t[t].data();
-                if (n == "DynamicGreenFunctions") {
-                    for (int k = 0; k < nk; ++k) {
-                        double acc = 0.0;
-                        for (int i = 0; i < ns; ++i)
-                            for (int j = 0; j < ns; ++j)
-                                acc += lat.fourier_re(lat.displacement[(size_t)i * ns + j], momentum_list[k]) * g[j + (size_t)i * ns] / ns;
-                        o.cache[(size_t)k * nt + t] += acc;
-                    }
-                } else if (n == "LocalDensityOfStates") {
-                    double tr = 0.0;
-                    for (int i = 0; i < ns; ++i) tr += g[i + (size_t)i * ns];
-                    o.cache[t] += tr / ns;
-                } else {
-                    double acc = 0.0;
-                    for (int i = 0; i < ns; ++i) {
-                        const double gg = g[i + (size_t)i * ns], hh = h[i + (size_t)i * ns];
-                        acc -= hh * gg + hh * gg;
-                    }
-                    o.cache[t] += acc / (2 * ns);
-                }
-            }
-            o.count++;
         }
     }
 

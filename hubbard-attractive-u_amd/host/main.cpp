@@ -147,6 +147,7 @@ int main(int argc, char* argv[])
             std::cout << ">> Configurations of ising fields initialized randomly.\n" << std::endl;
         }
         core.initialize_svdstacks(model);
+        core.setup_measurement(meas[0], lattice);
         core.initialize_green_functions();
         std::cout << ">> Initialization finished.\n\n>> The simulation is going to get started with parameters below:\n"
                   << std::endl;

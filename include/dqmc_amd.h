@@ -66,6 +66,8 @@ enum {
  * initialize() calls (src/main.cpp:119-128; src/dqmc_core.cpp:20-30; src/hubbard.cpp:25-48). */
 int dqmc_create(const dqmc_params_c* params, int n_chains, int precision, int delay_k, dqmc_ctx** out);
 void dqmc_destroy(dqmc_ctx* ctx);
+/* Message of the last failure of ctx; with ctx == NULL, of the calling thread's last failed
+ * call that has no context (dqmc_create before the context exists). */
 const char* dqmc_last_error(dqmc_ctx* ctx);
 
 /* Switch on the storage of G(t,t) / G(t,0), G(0,t) for measurements after creation (the
@@ -98,6 +100,15 @@ int dqmc_init(dqmc_ctx* ctx);
  * (include/hubbard.h:58-62, src/hubbard.cpp:293-341) for ops 0..4, applied in place to an
  * N x N column-major FP64 matrix with chain `chain`'s fields of slice t (FP64 path). */
 int dqmc_apply_bop(dqmc_ctx* ctx, int chain, int op, int t, double* inout);
+
+/* Single-site operations on one chain between sweeps (FP64 contexts), on its current G and fields:
+ * dqmc_acceptance_ratio replaces Model::HubbardAttractiveU::get_acceptance_ratio
+ * (src/hubbard.cpp:123-130), dqmc_update_green_function replaces update_green_function (:132-139,
+ * G for the flip of s(t,i), the field itself unchanged) and dqmc_flip_field replaces
+ * locally_update_ising_field (:118-121).  The sweeps apply the same three steps internally. */
+int dqmc_acceptance_ratio(dqmc_ctx* ctx, int chain, int t, int i, double* ratio);
+int dqmc_update_green_function(dqmc_ctx* ctx, int chain, int t, int i);
+int dqmc_flip_field(dqmc_ctx* ctx, int chain, int t, int i);
 
 /* Replaces Core::set_thermalization (include/dqmc_core.h:90). */
 int dqmc_set_thermalization(dqmc_ctx* ctx, int is_thermalization);
@@ -195,6 +206,36 @@ enum {
 };
 int dqmc_set_watchdog(dqmc_ctx* ctx, double max_dG);
 int dqmc_get_health(dqmc_ctx* ctx, int chain, uint32_t* flags);
+
+/* Observables on the device, from the stored Green's functions of every slice.
+ * Replace Measurement::Handle::initialize / measure_equaltime_observables /
+ * measure_dynamic_observables / normalize_cache + push_cache_to_data + clear_cache
+ * (src/measure_handle.cpp:18-104) and the estimators of Observable::Methods
+ * (include/observable_methods.hpp:39-195), for every chain of the context (each chain keeps its
+ * own bins, as each MPI rank of the reference does).
+ *   obs_mask: OR of the DQMC_OBS_* bits.  Equal-time observables need dqmc_set_collect(eqtime),
+ *             dynamic ones dqmc_set_collect(dynamic), both before dqmc_measure_setup.
+ *   nk, kxy:  the momentum list of DynamicGreenFunctions, nk (kx, ky) pairs (the k-stars of
+ *             src/square_lattice.cpp:83-95 selected by the momentum_list key).
+ * dqmc_measure_equaltime reads G(t,t) of all slices stored by the last updating sweep;
+ * dqmc_measure_dynamic reads G(t,0), G(0,t) of the last dynamic sweep.  dqmc_measure_push_bin
+ * divides each cache by its sample count, stores it as bin `bin` and clears it (an empty cache is
+ * the reference's "cache divided by zero" error).  dqmc_get_bins copies one chain's bins of one
+ * observable: [nbins] for the scalars, [nbins][nk][nt] for DQMC_OBS_GREEN_KT, [nbins][nt] for
+ * DQMC_OBS_LDOS and DQMC_OBS_SPIN. */
+enum {
+    DQMC_OBS_FILLING = 1,       /* FillingNumber                         */
+    DQMC_OBS_DOUBLE_OCCU = 2,   /* DoubleOccupation                      */
+    DQMC_OBS_SWAVE = 4,         /* StaticSWavePairingCorrelation         */
+    DQMC_OBS_GREEN_KT = 8,      /* DynamicGreenFunctions                 */
+    DQMC_OBS_LDOS = 16,         /* LocalDensityOfStates                  */
+    DQMC_OBS_SPIN = 32          /* LocalDynamicTransverseSpinCorrelation */
+};
+int dqmc_measure_setup(dqmc_ctx* ctx, unsigned obs_mask, int nbins, int nk, const double* kxy);
+int dqmc_measure_equaltime(dqmc_ctx* ctx);
+int dqmc_measure_dynamic(dqmc_ctx* ctx);
+int dqmc_measure_push_bin(dqmc_ctx* ctx, int bin);
+int dqmc_get_bins(dqmc_ctx* ctx, int chain, unsigned obs, double* out);
 
 /* Lattice size / chain count accessors. */
 int dqmc_dims(dqmc_ctx* ctx, int* ns, int* nt, int* n_chains);

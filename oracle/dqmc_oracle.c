@@ -938,6 +938,10 @@ int orc_get_gf(orc_ctx* c, int kind, int t, double* out)
     return -1;
 }
 void orc_set_gftt(orc_ctx* c, const double* g) { mat_copy(c->n, c->gftt, g); }
+/* get_acceptance_ratio / update_green_function / locally_update_ising_field on the current G */
+double orc_acceptance_ratio(orc_ctx* c, int t, int i) { return accept_ratio(c, t, i); }
+void orc_update_green_function(orc_ctx* c, int t, int i) { sm_update(c, t, i); }
+void orc_flip_field(orc_ctx* c, int t, int i) { c->fields[t * c->n + i] = -c->fields[t * c->n + i]; }
 /* matrix_compare_error itself, exported for its own tests */
 double orc_compare_error(int n, const double* a, const double* b) { return compare_error(n, a, b); }
 void orc_get_stats(orc_ctx* c, double* wrap_eq, double* wrap_dyn, long long* accepts, long long* visits)

@@ -63,6 +63,9 @@ def lib():
             "orc_get_gf": (C.c_int, [vp, C.c_int, C.c_int, _dp]),
             "orc_set_gftt": (None, [vp, _dp]),
             "orc_compare_error": (C.c_double, [C.c_int, _dp, _dp]),
+            "orc_acceptance_ratio": (C.c_double, [vp, C.c_int, C.c_int]),
+            "orc_update_green_function": (None, [vp, C.c_int, C.c_int]),
+            "orc_flip_field": (None, [vp, C.c_int, C.c_int]),
             "orc_get_stats": (None, [vp, _dp, _dp, _llp, _llp]),
             "orc_get_accmask": (None, [vp, _u32p]),
             "orc_current_t": (C.c_int, [vp]),
@@ -229,6 +232,15 @@ class Chain:
         if rc:
             raise RuntimeError("gf kind %d not collected" % kind)
         return out
+
+    def acceptance_ratio(self, t, i):
+        return lib().orc_acceptance_ratio(self._c, t, i)
+
+    def update_green_function(self, t, i):
+        lib().orc_update_green_function(self._c, t, i)
+
+    def flip_field(self, t, i):
+        lib().orc_flip_field(self._c, t, i)
 
     def set_gftt(self, g):
         g = np.asfortranarray(g, dtype=np.float64)

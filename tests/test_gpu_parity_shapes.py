@@ -264,3 +264,28 @@ def test_mixed_precision_meets_bound_at_benchmark_shape():
     say("  mixed max|G - G_FP64| = %.3e" % dg)
     e.close()
     assert dg <= 1e-6
+
+
+def test_single_site_operations_match_oracle():
+    # get_acceptance_ratio / update_green_function / locally_update_ising_field through the ABI,
+    # the reference's Metropolis step done by hand on slice 0 of the initial G
+    nl, nt, beta, u, mu, pace = 6, 20, 2.0, 4.0, -0.3, 5
+    say("case: single-site operations nl=6")
+    o = oracle_chain(nl, nt, beta, u, mu, pace, 1234)
+    e = gpu.Engine(nl, nt, beta, u, mu=mu, pace=pace, n_chains=2, delay_k=1)
+    for c in range(2):
+        e.set_seed(c, 1234)
+        e.fields_random(c)
+    e.init()
+    for i in (0, 7, 35, 7):
+        r_o, r_e = o.acceptance_ratio(0, i), e.acceptance_ratio(1, 0, i)
+        assert abs(r_e - r_o) <= 1e-12 * abs(r_o)
+        o.update_green_function(0, i)
+        o.flip_field(0, i)
+        e.update_green_function(1, 0, i)
+        e.flip_field(1, 0, i)
+        assert np.abs(e.gf(1) - o.gf(0)).max() < TOL_G
+    assert np.array_equal(e.fields(1).astype(np.int32), o.fields)
+    assert not np.array_equal(e.fields(0), e.fields(1))          # chain 0 untouched
+    o.close()
+    e.close()
