@@ -86,6 +86,7 @@ def lib():
             "dqmc_forward_fixed": (C.c_int, [vp, dp, dp]),
             "dqmc_adjoint": (C.c_int, [vp, dp, dp, dp]),
             "dqmc_dims": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+            "dqmc_apply_bop_noalloc": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int, vp, vp, vp]),
             "dqmc_measure_setup": (C.c_int, [vp, C.c_uint, C.c_int, C.c_int, dp]),
             "dqmc_measure_equaltime": (C.c_int, [vp]),
             "dqmc_measure_dynamic": (C.c_int, [vp]),
@@ -103,6 +104,18 @@ def lib():
             f.argtypes = args
         _lib = L
     return _lib
+
+
+def apply_bop_noalloc(nl, nt, beta, u, mu, nnt, n_chains, op, d_fields, d_inout, stream=None):
+    """dqmc_apply_bop_noalloc on device memory the caller owns (addresses as ints): queues
+    operation `op` of Engine.apply_bop on `stream` for n_chains column-major N x N matrices at
+    d_inout, with one slice of fields (int8, [chain][N]) at d_fields.  Allocates nothing."""
+    p = Params(nl, nt, beta, nnt, u, mu, 1, 0, 0, 0)
+    rc = lib().dqmc_apply_bop_noalloc(C.byref(p), n_chains, op, C.c_void_p(d_fields), C.c_void_p(d_inout),
+                                      C.c_void_p(stream) if stream else None)
+    if rc:
+        raise DqmcError("dqmc_apply_bop_noalloc failed: %s %s" % (_ERRS.get(rc, rc), lib().dqmc_last_error(None).decode()),
+                        rc)
 
 
 class Engine:

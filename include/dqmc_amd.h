@@ -67,7 +67,7 @@ enum {
 int dqmc_create(const dqmc_params_c* params, int n_chains, int precision, int delay_k, dqmc_ctx** out);
 void dqmc_destroy(dqmc_ctx* ctx);
 /* Message of the last failure of ctx; with ctx == NULL, of the calling thread's last failed
- * call that has no context (dqmc_create before the context exists). */
+ * call that has no context (dqmc_create before the context exists, dqmc_apply_bop_noalloc). */
 const char* dqmc_last_error(dqmc_ctx* ctx);
 
 /* Switch on the storage of G(t,t) / G(t,0), G(0,t) for measurements after creation (the
@@ -236,6 +236,23 @@ int dqmc_measure_equaltime(dqmc_ctx* ctx);
 int dqmc_measure_dynamic(dqmc_ctx* ctx);
 int dqmc_measure_push_bin(dqmc_ctx* ctx, int bin);
 int dqmc_get_bins(dqmc_ctx* ctx, int chain, unsigned obs, double* out);
+
+/* Allocation-free B-operations.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).  Applies operation
+ * `op` of dqmc_apply_bop (0: B X, 1: X B, 2: B^{-1} X, 3: X B^{-1}, 4: B^T X, with
+ * B = diag(e^{dt mu + alpha s}) exp(dt nnt A) of include/hubbard.h:58-62) in place to n_chains
+ * N x N column-major FP64 matrices in device memory d_inout (chain c at c * N * N), with the fields
+ * of one slice in device memory d_fields (int8 +1/-1, chain c at c * N).  params supplies nl (<= 16),
+ * nt, beta, nnt, u and mu (one mu for all chains).  Work is queued on `stream` (a hipStream_t;
+ * NULL = the default stream) and the call returns without waiting; it calls no hipMalloc and no
+ * allocating routine.  Intermediates live in a library-owned device array of
+ * DQMC_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: n_chains * N * N * 8 bytes
+ * must fit in it beside 8 KiB of constants, else DQMC_EINVAL.  All calls on one GPU share that
+ * array, so calls that overlap on one GPU (different streams or threads) race.  Results equal
+ * dqmc_apply_bop's.  Errors are reported by dqmc_last_error(NULL) of the calling thread. */
+#define DQMC_NOALLOC_SCRATCH_BYTES (32LL << 20)
+int dqmc_apply_bop_noalloc(const dqmc_params_c* params, int n_chains, int op, const int8_t* d_fields,
+                           double* d_inout, void* stream);
 
 /* Lattice size / chain count accessors. */
 int dqmc_dims(dqmc_ctx* ctx, int* ns, int* nt, int* n_chains);
