@@ -69,6 +69,7 @@ struct dqmc_ctx {
     bool kron = true;
     bool trsm_v1 = false;   // DQMC_TRSM=v1: one-kernel thread-per-column back-substitution
     int qr_update = QR_UPDATE_REG;   // QR trailing update (DQMC_QR_UPDATE=gemm: three GEMMs)
+    int qr_panel = QR_PANEL_256;     // QR panel: 4 waves per chain (DQMC_QR_PANEL=1024: 16 waves)
     int metro_threads = 1024;  // Metropolis workgroup size at N = 256 (DQMC_METRO_THREADS)
     double *qrV = nullptr, *qrT = nullptr, *qrWt = nullptr;   // blocked-QR scratch: V, T, Wt and Wt T
     int current_t = 0;
@@ -256,7 +257,8 @@ int gemm(dqmc_ctx* c, const T* A, long long sA, int lda, int tA, const T* B, lon
 hipError_t qr_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int batch, hipStream_t s)
 {
     if (c->qr_mode == 1) return launch_qr_v1(buf, stride, N, W, batch, s);
-    return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrT, c->qrWt, 3LL * N * 32, c->tile64, c->qr_update, s);
+    return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrT, c->qrWt, 3LL * N * 32, c->tile64, c->qr_update,
+                             c->qr_panel, s);
 }
 hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int c0, int P, int batch, hipStream_t s)
 {
@@ -806,6 +808,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     {
         const char* q = std::getenv("DQMC_QR");
         c->qr_mode = (q && std::string(q) == "v1") ? 1 : 0;
+        const char* qp = std::getenv("DQMC_QR_PANEL");
+        if (qp && std::string(qp) == "1024") c->qr_panel = QR_PANEL_1024;
         const char* qu = std::getenv("DQMC_QR_UPDATE");
         if (qu && std::string(qu) == "gemm") c->qr_update = QR_UPDATE_GEMM;
         const char* mt = std::getenv("DQMC_METRO_THREADS");

@@ -115,9 +115,12 @@ hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, 
 // `update`: QR_UPDATE_REG (default) applies the whole block reflector in one launch per panel with
 // each 32-column tile held in registers (the first Wt block holds a row-major copy of V);
 // QR_UPDATE_GEMM uses three MFMA GEMM launches with tile `tile` for the last one.
+// `panel`: QR_PANEL_256 (default) factors each panel with 4 waves per chain, a thread per row;
+// QR_PANEL_1024 with 16 waves, four threads per row.
 enum QrUpdate : int { QR_UPDATE_GEMM = 0, QR_UPDATE_REG = 2 };
+enum QrPanel : int { QR_PANEL_256 = 0, QR_PANEL_1024 = 1 };
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
-                             double* Wt, long long wt_stride, int tile, int update, hipStream_t s);
+                             double* Wt, long long wt_stride, int tile, int update, int panel, hipStream_t s);
 // Blocked in-place back-substitution of the same system: 32-row diagonal solves (thread per
 // right-hand side) and MFMA GEMM updates of the rows above (`tile` = GEMM tile).
 hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,

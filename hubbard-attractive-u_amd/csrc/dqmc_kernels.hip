@@ -1321,6 +1321,162 @@ __global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bu
     }
 }
 
+// Sum every one of the 32 values d[0..31] over the 64 lanes of a wave by recursive halving, with
+// register-only exchanges: at each level a lane keeps the half of its values selected by one lane
+// bit and adds the same half of the partner lane that differs in that bit.  For bits 5 and 4 one
+// permlane32 / permlane16 swap of the pair (low half, high half) hands every lane its own kept value
+// and the partner's in the two results, so their sum needs no select; bits 3 and 2 use the DPP row
+// mirror and half-row mirror, bits 1 and 0 quad perms.  On exit lane l holds the wave sum of value
+// 16 b5 + 8 b4 + 4 b3 + 2 b2 + b1 (b_k = bit k of l), identical in lanes l and l ^ 1.
+template <bool ROW16>
+__device__ __forceinline__ double swap_sum_f64(double lo_half, double hi_half)
+{
+    const unsigned long long a = (unsigned long long)__double_as_longlong(lo_half);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(hi_half);
+    unsigned r0l, r1l, r0h, r1h;
+    if (ROW16) {
+        const auto l = __builtin_amdgcn_permlane16_swap((unsigned)a, (unsigned)b, false, false);
+        const auto h = __builtin_amdgcn_permlane16_swap((unsigned)(a >> 32), (unsigned)(b >> 32), false, false);
+        r0l = l[0]; r1l = l[1]; r0h = h[0]; r1h = h[1];
+    } else {
+        const auto l = __builtin_amdgcn_permlane32_swap((unsigned)a, (unsigned)b, false, false);
+        const auto h = __builtin_amdgcn_permlane32_swap((unsigned)(a >> 32), (unsigned)(b >> 32), false, false);
+        r0l = l[0]; r1l = l[1]; r0h = h[0]; r1h = h[1];
+    }
+    const double r0 = __longlong_as_double((long long)(((unsigned long long)r0h << 32) | r0l));
+    const double r1 = __longlong_as_double((long long)(((unsigned long long)r1h << 32) | r1l));
+    return r0 + r1;
+}
+__device__ __forceinline__ double wave_halve32(const double (&d)[32], int lane)
+{
+    double a16[16], a8[8], a4[4], a2[2];
+    const bool h8 = lane & 8, h4 = lane & 4, h2 = lane & 2;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) a16[m] = swap_sum_f64<false>(d[m], d[m + 16]);
+#pragma unroll
+    for (int m = 0; m < 8; ++m) a8[m] = swap_sum_f64<true>(a16[m], a16[m + 8]);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        a4[m] = (h8 ? a8[m + 4] : a8[m]) + dpp_f64<DPP_MIRROR>(h8 ? a8[m] : a8[m + 4]);
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+        a2[m] = (h4 ? a4[m + 2] : a4[m]) + dpp_f64<DPP_HALF_MIRROR>(h4 ? a4[m] : a4[m + 2]);
+    double a1 = (h2 ? a2[1] : a2[0]) + dpp_f64<DPP_XOR2>(h2 ? a2[0] : a2[1]);
+    return a1 + dpp_f64<DPP_XOR1>(a1);
+}
+__device__ __forceinline__ int halve32_index(int lane)
+{
+    return ((lane & 32) ? 16 : 0) + ((lane & 16) ? 8 : 0) + ((lane & 8) ? 4 : 0) + ((lane & 4) ? 2 : 0) +
+           ((lane & 2) ? 1 : 0);
+}
+
+// Panel factorization, 256 threads (4 waves) per chain: thread r holds panel row r (rows p0 .. N-1,
+// so N - p0 <= 256) with all QRB columns in registers.  Column i costs two 4-wave barriers: one
+// after the partial norms (alpha = x(i,i) is published with them), one after the partial sums of
+// the 32 products v . x_j (j > i) and v_k . v (k < i, the V^T V entries of T) that one recursive
+// halving reduces per wave.  The column sums reach every lane through v_readlane (uniform values).
+// Every LDS slot is written between the second barrier of one column and the first of the next,
+// and read between the two barriers of the column that wrote it, so single buffers suffice.
+// T (H_1..H_nb = I - V T V^T, column-major QRB x QRB) comes from the recursion
+// T(j,i) = -tau_i sum_{k=j}^{i-1} T(j,k) (V^T V)(k,i), thread j forming row j on its own.
+__global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs, long long stride, int N, int W,
+                                                       int p0, int nb, double* __restrict__ Vs,
+                                                       double* __restrict__ Ts, double* __restrict__ Vrows,
+                                                       long long vr_stride)
+{
+    __shared__ double nrm[4];
+    __shared__ double alph;
+    __shared__ double dot[4][QRB];
+    __shared__ __align__(16) double VtVt[QRB][QRB];   // VtVt[i][k] = (V^T V)(k, i), k < i
+    __shared__ double tau_s[QRB];
+    const int r = threadIdx.x, lane = r & 63, wave = r >> 6;
+    const int b = blockIdx.x;
+    const int R = N - p0;
+    const bool live = r < R;
+    double* A = bufs + (long long)b * stride;
+    double* V = Vs + (long long)b * N * QRB;
+    double* Tg = Ts + (long long)b * QRB * QRB;
+
+    double x[QRB];
+#pragma unroll
+    for (int m = 0; m < QRB; ++m) x[m] = (live && m < nb) ? A[(long long)(p0 + r) * W + p0 + m] : 0.0;
+    if (r < QRB) tau_s[r] = 0.0;
+    for (int e = r; e < QRB * QRB; e += 256) VtVt[e / QRB][e % QRB] = 0.0;
+
+#pragma clang loop unroll(full)
+    for (int i = 0; i < QRB; ++i) {
+        if (i >= nb) continue;   // uniform
+        const double xi = x[i];
+        if (r == i) alph = xi;
+        double n2 = (live && r > i) ? xi * xi : 0.0;
+        n2 = wave_sum_dpp(n2);
+        if (lane == 0) nrm[wave] = n2;
+        __syncthreads();
+        const double s2 = ((nrm[0] + nrm[1]) + nrm[2]) + nrm[3];
+        const double alpha = alph;
+        double beta = alpha, tau = 0.0, scal = 0.0;
+        if (s2 > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        const double v = (r == i) ? 1.0 : ((live && r > i) ? xi * scal : 0.0);
+        if (r == i) x[i] = beta;
+        else if (r > i) x[i] = v;
+        // m < i: v_m . v_i (V^T V (m, i)); m > i: v_i . x_m.  Rows r < i have v = 0, and rows r >= i
+        // hold v_m(r) in x[m] for m < i; columns m >= nb stay zero, so no product needs a guard.
+        double d[QRB];
+#pragma unroll
+        for (int m = 0; m < QRB; ++m) d[m] = (m == i) ? 0.0 : v * x[m];
+        const double ws = wave_halve32(d, lane);
+        if ((lane & 1) == 0) dot[wave][halve32_index(lane)] = ws;
+        if (r == 0) tau_s[i] = tau;
+        __syncthreads();
+        const double wsum = (lane < QRB) ? ((dot[0][lane] + dot[1][lane]) + dot[2][lane]) + dot[3][lane] : 0.0;
+        if (r < i) VtVt[i][r] = wsum;
+        const double tv = tau * v;
+#pragma unroll
+        for (int m = 0; m < QRB; ++m)
+            if (m > i) x[m] -= tv * readlane_f64(wsum, m);
+    }
+    __syncthreads();
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < QRB; ++m) {
+            if (m < nb) {
+                A[(long long)(p0 + r) * W + p0 + m] = (r <= m) ? x[m] : 0.0;
+                V[r + (long long)m * N] = (r == m) ? 1.0 : (r > m ? x[m] : 0.0);
+            }
+        }
+        if (Vrows) {
+            double* vr = Vrows + (long long)b * vr_stride + (long long)r * QRB;
+#pragma unroll
+            for (int m = 0; m < QRB; ++m)
+                if (m < nb) vr[m] = (r == m) ? 1.0 : (r > m ? x[m] : 0.0);
+        }
+    }
+    if (r < QRB) {
+        // row j of T: T(j, k) = 0 for k < j, so the sums run over all k < i without a guard
+        // (tau_i = 0 for i >= nb leaves those columns zero)
+        const int j = r;
+        double tr[QRB];
+#pragma unroll
+        for (int i = 0; i < QRB; ++i) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < i; k += 2) {
+                const double2 vv = *reinterpret_cast<const double2*>(&VtVt[i][k]);
+                s += tr[k] * vv.x;
+                if (k + 1 < i) s += tr[k + 1] * vv.y;
+            }
+            const double t = (i == j) ? tau_s[i] : -tau_s[i] * s;
+            tr[i] = (i < j) ? 0.0 : t;
+        }
+#pragma unroll
+        for (int k = 0; k < QRB; ++k) Tg[j + (long long)k * QRB] = tr[k];
+    }
+}
+
 // Fused trailing update of one panel: every trailing column a of the panel rows (right-hand
 // sides included) becomes a - V T^T V^T a, i.e. a - V (a^T V T)^T.  One workgroup takes 32
 // trailing columns of one chain and holds them in registers.  256 threads; wave w owns the
@@ -1448,7 +1604,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 }
 
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
-                             double* Wt, long long wt_stride, int tile, int update, hipStream_t s)
+                             double* Wt, long long wt_stride, int tile, int update, int panel, hipStream_t s)
 {
     // per chain: Vs N*QRB, Ts QRB*QRB, Wt 2 * wt_stride (Wt and Wt T; in QR_UPDATE_REG the first
     // holds the row-major copy of V), wt_stride >= W*QRB
@@ -1459,8 +1615,12 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
         const int nb = (N - p0 < QRB) ? N - p0 : QRB;
         const int R = N - p0, cols = W - (p0 + nb);
         const bool reg = update == QR_UPDATE_REG;
-        hipLaunchKernelGGL(qr_panel4_kernel, dim3(batch), dim3(1024), 0, s, buf, stride, N, W, p0, nb, Vs, Ts,
-                           reg ? Wt : nullptr, wt_stride);
+        if (panel == QR_PANEL_1024)
+            hipLaunchKernelGGL(qr_panel4_kernel, dim3(batch), dim3(1024), 0, s, buf, stride, N, W, p0, nb, Vs, Ts,
+                               reg ? Wt : nullptr, wt_stride);
+        else
+            hipLaunchKernelGGL(qr_panel_kernel, dim3(batch), dim3(256), 0, s, buf, stride, N, W, p0, nb, Vs, Ts,
+                               reg ? Wt : nullptr, wt_stride);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (cols <= 0) continue;
