@@ -70,6 +70,8 @@ struct dqmc_ctx {
     bool trsm_v1 = false;   // DQMC_TRSM=v1: one-kernel thread-per-column back-substitution
     int qr_update = QR_UPDATE_REG;   // QR trailing update (DQMC_QR_UPDATE=gemm: three GEMMs)
     int qr_panel = QR_PANEL_256;     // QR panel: 4 waves per chain (DQMC_QR_PANEL=1024: 16 waves)
+    bool solve_lu = true;            // recompose solve by partial-pivot LU (DQMC_SOLVE=qr: Householder QR)
+    int* lu_perm = nullptr;          // [B][N] row order of the LU panel rows
     int metro_threads = 1024;  // Metropolis workgroup size at N = 256 (DQMC_METRO_THREADS)
     double *qrV = nullptr, *qrT = nullptr, *qrWt = nullptr;   // blocked-QR scratch: V, T, Wt and Wt T
     int current_t = 0;
@@ -266,6 +268,16 @@ hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, in
     return launch_trsm_blocked(buf, stride, N, W, c0, P, batch, c->tile64, s);
 }
 
+// Factorization of the recompose system [temp | rhs] (row-major N x W in c->qr): partial-pivot LU
+// like the reference's PartialPivLU (stable_numerics.hpp:235, 304, 322), or Householder QR.  Either
+// leaves an upper triangle for the back-substitution and the transformed right-hand sides.
+hipError_t factor_any(dqmc_ctx* c, int W)
+{
+    if (c->solve_lu)
+        return launch_lu_blocked(c->qr, c->sqr(), c->N, W, c->B, c->qrV, c->qrT, c->lu_perm, c->health, c->stream);
+    return qr_any(c, c->qr, c->sqr(), c->N, W, c->B, c->stream);
+}
+
 // C = diag(r) (E(x)E) diag(s) X diag(cv)  (left) or  diag(r) X diag(s) (E(x)E) diag(cv)  (right)
 template <typename T>
 int kron(dqmc_ctx* c, bool left, const T* X, T* C, const T* E, DiagArg rowd = DiagArg(), DiagArg innd = DiagArg(),
@@ -442,7 +454,7 @@ int recompute_eq(dqmc_ctx* c, double* out)
     RCHECK(gemm<double>(c, Vl, st, N, 1, Vr, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_MIN1, Dl, sd), dnone(),
                         dvec(DIAG_MIN1, Dr, sd), 1.0, 1.0));
     HCHECK(c, launch_build_rhs(Ul, st, dvec(DIAG_INVMAX1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, N, c->B, c->stream));
-    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, W, c->B, c->stream));
+    TIMED(c, KT_QR, factor_any(c, W));
     TIMED(c, KT_TRSM, trsm_any(c, c->qr, c->sqr(), N, W, N, N, c->B, c->stream));
     // G = Ur Drmax^{-1} X
     RCHECK(gemm<double>(c, Ur, st, N, 0, c->qr + N, c->sqr(), W, 1, out, nn, N, 0, dnone(), dvec(DIAG_INVMAX1, Dr, sd)));
@@ -476,7 +488,7 @@ int recompute_dyn(dqmc_ctx* c, double* gtt, double* gt0n, double* g0tn)
                         dvec(DIAG_MIN1, Dr, sd), 1.0, 1.0));
     HCHECK(c, launch_build_rhs(Ul, st, dvec(DIAG_INVMAX1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, N, c->B, c->stream));
     HCHECK(c, launch_build_rhs(Vl, st, dvec(DIAG_MIN1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, 2 * N, c->B, c->stream));
-    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, W, c->B, c->stream));
+    TIMED(c, KT_QR, factor_any(c, W));
     TIMED(c, KT_TRSM, trsm_any(c, c->qr, c->sqr(), N, W, N, 2 * N, c->B, c->stream));
     RCHECK(gemm<double>(c, Ur, st, N, 0, c->qr + N, c->sqr(), W, 1, gtt, nn, N, 0, dnone(), dvec(DIAG_INVMAX1, Dr, sd)));
     RCHECK(gemm<double>(c, Ur, st, N, 0, c->qr + 2 * N, c->sqr(), W, 1, gt0n, nn, N, 0, dnone(),
@@ -808,6 +820,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     {
         const char* q = std::getenv("DQMC_QR");
         c->qr_mode = (q && std::string(q) == "v1") ? 1 : 0;
+        const char* sv = std::getenv("DQMC_SOLVE");
+        c->solve_lu = !(sv && std::string(sv) == "qr");
         const char* qp = std::getenv("DQMC_QR_PANEL");
         if (qp && std::string(qp) == "1024") c->qr_panel = QR_PANEL_1024;
         const char* qu = std::getenv("DQMC_QR_UPDATE");
@@ -855,6 +869,7 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     RCHECK(c->alloc(&c->W3, (size_t)B * nn * 8));
     RCHECK(c->alloc(&c->qr, (size_t)B * c->sqr() * 8));
     RCHECK(c->alloc(&c->perm, (size_t)B * N * 4));
+    RCHECK(c->alloc(&c->lu_perm, (size_t)B * N * 4));
     RCHECK(c->alloc(&c->qrV, (size_t)B * N * 32 * 8)); RCHECK(c->alloc(&c->qrT, (size_t)B * 32 * 32 * 8));
     RCHECK(c->alloc(&c->qrWt, (size_t)2 * B * 3 * N * 32 * 8));
     for (DStack* s : {&c->left, &c->right}) {

@@ -121,6 +121,12 @@ enum QrUpdate : int { QR_UPDATE_GEMM = 0, QR_UPDATE_REG = 2 };
 enum QrPanel : int { QR_PANEL_256 = 0, QR_PANEL_1024 = 1 };
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
                              double* Wt, long long wt_stride, int tile, int update, int panel, hipStream_t s);
+// Blocked LU with partial pivoting of the leading N x N block of the same row-major N x W system
+// (N <= 256), applied to the W - N trailing columns: on exit the leading block holds U and the
+// trailing columns L^{-1} P rhs.  Scratch per chain: Ls N*32 doubles, Li 32*32 doubles, Ps N ints.
+// A zero pivot sets HEALTH_ZERO_PIVOT.
+hipError_t launch_lu_blocked(double* buf, long long stride, int N, int W, int batch, double* Ls, double* Li, int* Ps,
+                             uint32_t* health, hipStream_t s);
 // Blocked in-place back-substitution of the same system: 32-row diagonal solves (thread per
 // right-hand side) and MFMA GEMM updates of the rows above (`tile` = GEMM tile).
 hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,

@@ -1664,6 +1664,269 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
 }
 
 // ---------------------------------------------------------------------------------------------
+//  Blocked LU with partial pivoting of the leading N x N block of the row-major N x W system,
+//  applied to the W - N trailing right-hand-side columns (the reference inverts this matrix with
+//  Eigen's PartialPivLU, include/utils/stable_numerics.hpp:235, 304, 322).  Per panel of QRB columns:
+//    lu_panel_kernel   4 waves per chain, thread r holds panel row r: per column an arg-max
+//                      reduction (ties to the lower row, as LAPACK's idamax), the swap with the
+//                      pivot row and the rank-1 elimination; writes L and U back, L column-major
+//                      (Ls), the 32 x 32 inverse of the unit lower L11 (Li) and the row order of
+//                      the panel rows (Ps).
+//    lu_update_kernel  one workgroup per 32 trailing columns: loads the tile in the panel's row
+//                      order, U12 = L11^{-1} A12 and A22 -= L21 U12 on MFMA.
+//  On exit the leading block holds U (upper, with its diagonal) and the trailing columns hold
+//  L^{-1} P rhs, so back-substitution with U finishes the solve.
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int xchg_i32(int v, int lane, bool row16)
+{
+    const auto r = row16 ? __builtin_amdgcn_permlane16_swap((unsigned)v, (unsigned)v, false, false)
+                         : __builtin_amdgcn_permlane32_swap((unsigned)v, (unsigned)v, false, false);
+    const bool upper = row16 ? (lane & 16) : (lane & 32);
+    return (int)(upper ? r[0] : r[1]);
+}
+__device__ __forceinline__ double xchg_f64(double v, int lane, bool row16)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)xchg_i32((int)(unsigned)u, lane, row16);
+    const unsigned hi = (unsigned)xchg_i32((int)(unsigned)(u >> 32), lane, row16);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// larger value wins, equal values go to the smaller index
+__device__ __forceinline__ void argmax_take(double& v, int& i, double ov, int oi)
+{
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+// (value, index) arg-max over the 64 lanes; every lane ends with the result.  The pairings (quad
+// perms, half-row and row mirrors, row-pair and half swaps) each join two groups that already
+// agree, so after the six levels the whole wave agrees.
+__device__ __forceinline__ void wave_argmax(double& v, int& i, int lane)
+{
+    argmax_take(v, i, dpp_f64<DPP_XOR1>(v), __builtin_amdgcn_mov_dpp(i, DPP_XOR1, 0xF, 0xF, false));
+    argmax_take(v, i, dpp_f64<DPP_XOR2>(v), __builtin_amdgcn_mov_dpp(i, DPP_XOR2, 0xF, 0xF, false));
+    argmax_take(v, i, dpp_f64<DPP_HALF_MIRROR>(v), __builtin_amdgcn_mov_dpp(i, DPP_HALF_MIRROR, 0xF, 0xF, false));
+    argmax_take(v, i, dpp_f64<DPP_MIRROR>(v), __builtin_amdgcn_mov_dpp(i, DPP_MIRROR, 0xF, 0xF, false));
+    argmax_take(v, i, xchg_f64(v, lane, true), xchg_i32(i, lane, true));
+    argmax_take(v, i, xchg_f64(v, lane, false), xchg_i32(i, lane, false));
+}
+
+// One barrier per column: before it every wave publishes its arg-max (value, row) and that row
+// (values and original index), and row i publishes itself; after it every thread picks the
+// winning wave, row i takes the pivot row, the pivot's old position takes row i, and the rows
+// below eliminate with the pivot row (read by lane j from LDS, then v_readlane to every lane).
+// The published slots alternate between two buffers by column parity: a slot written for column
+// i + 2 is past the barrier of column i + 1, which every reader of column i reached after its reads.
+__global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs, long long stride, int N, int W,
+                                                       int p0, int nb, double* __restrict__ Ls,
+                                                       double* __restrict__ Li, int* __restrict__ Ps,
+                                                       uint32_t* __restrict__ health)
+{
+    __shared__ double redv[2][4];
+    __shared__ int redi[2][4];
+    __shared__ __align__(16) double cand[2][4][QRB];
+    __shared__ int cando[2][4];
+    __shared__ __align__(16) double irow[2][QRB];
+    __shared__ int irowo[2];
+    __shared__ double L11[QRB][QRB + 1];
+    const int r = threadIdx.x, lane = r & 63, wave = r >> 6;
+    const int b = blockIdx.x;
+    const int R = N - p0;
+    const bool live = r < R;
+    double* A = bufs + (long long)b * stride;
+
+    double x[QRB];
+#pragma unroll
+    for (int m = 0; m < QRB; ++m) x[m] = (live && m < nb) ? A[(long long)(p0 + r) * W + p0 + m] : 0.0;
+    int org = r;
+    bool zero_pivot = false;
+
+#pragma clang loop unroll(full)
+    for (int i = 0; i < QRB; ++i) {
+        if (i >= nb) continue;   // uniform
+        const int par = i & 1;
+        double cv = (live && r >= i) ? fabs(x[i]) : -1.0;
+        int ci = r;
+        wave_argmax(cv, ci, lane);
+        if (r == ci) {   // this lane holds its wave's candidate row
+#pragma unroll
+            for (int m = 0; m < QRB; m += 2) {
+                double2 v; v.x = x[m]; v.y = x[m + 1];
+                *reinterpret_cast<double2*>(&cand[par][wave][m]) = v;
+            }
+            cando[par][wave] = org;
+        }
+        if (lane == 0) { redv[par][wave] = cv; redi[par][wave] = ci; }
+        if (r == i) {
+#pragma unroll
+            for (int m = 0; m < QRB; m += 2) {
+                double2 v; v.x = x[m]; v.y = x[m + 1];
+                *reinterpret_cast<double2*>(&irow[par][m]) = v;
+            }
+            irowo[par] = org;
+        }
+        __syncthreads();
+        double bv = redv[par][0];
+        int bi = redi[par][0], bw = 0;
+#pragma unroll
+        for (int w = 1; w < 4; ++w) {
+            const double v = redv[par][w];
+            const int ix = redi[par][w];
+            if (v > bv || (v == bv && ix < bi)) { bv = v; bi = ix; bw = w; }
+        }
+        // pivot row to every lane: lane j reads entry j, v_readlane broadcasts (uniform values)
+        const double pj = (lane < QRB) ? cand[par][bw][lane] : 0.0;
+        if (r == i && bi != i) {
+#pragma unroll
+            for (int m = 0; m < QRB; ++m) x[m] = cand[par][bw][m];
+            org = cando[par][bw];
+        } else if (r == bi && bi != i) {
+#pragma unroll
+            for (int m = 0; m < QRB; ++m) x[m] = irow[par][m];
+            org = irowo[par];
+        }
+        const double piv = readlane_f64(pj, i);
+        zero_pivot |= !(piv != 0.0);
+        const double l = (live && r > i) ? x[i] / piv : 0.0;
+        if (r > i) x[i] = l;
+#pragma unroll
+        for (int m = 0; m < QRB; ++m)
+            if (m > i) x[m] -= l * readlane_f64(pj, m);
+    }
+    if (r == 0 && zero_pivot) atomicOr(health + b, HEALTH_ZERO_PIVOT);
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < QRB; ++m)
+            if (m < nb) {
+                A[(long long)(p0 + r) * W + p0 + m] = x[m];
+                Ls[(long long)b * N * QRB + r + (long long)m * N] = (r > m) ? x[m] : (r == m ? 1.0 : 0.0);
+            }
+        Ps[(long long)b * N + r] = org;
+    }
+    if (r < QRB) {
+#pragma unroll
+        for (int m = 0; m < QRB; ++m) L11[r][m] = (r > m && r < nb && m < nb) ? x[m] : 0.0;
+    }
+    __syncthreads();
+    if (r < QRB) {
+        // column j of L11^{-1} by forward substitution: y = 0 above j, y_j = 1, and the terms with
+        // q < j vanish, so the sums need no guard; columns j >= nb stay zero
+        const int j = r;
+        double y[QRB];
+#pragma unroll
+        for (int m = 0; m < QRB; ++m) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < m; ++q) s += L11[m][q] * y[q];
+            y[m] = (m < j || j >= nb) ? 0.0 : (m == j ? 1.0 : -s);
+        }
+#pragma unroll
+        for (int m = 0; m < QRB; ++m) Li[(long long)b * QRB * QRB + m + (long long)j * QRB] = y[m];
+    }
+}
+
+// Trailing update of one LU panel on a 32-column tile, 256 threads: wave w holds the 16-row tiles
+// rt = w, w + 4, w + 8, w + 12 of the tile, rows loaded in the panel's row order (Ps) straight into
+// f64 accumulator layout (item r of tile (rt, ct) = At(16 rt + 4 r + l/16, 16 ct + l%16)).  Waves 0
+// and 1 hold the QRB panel rows A12: through LDS they form U12 = L11^{-1} A12, which every wave then
+// reads as B-fragments for A22 -= L21 U12 on its own row tiles.  The tile is read and written once.
+__global__ __launch_bounds__(256) void lu_update_kernel(double* __restrict__ bufs, long long stride, int N, int W,
+                                                        int batch, int p0, int nb, const double* __restrict__ Ls,
+                                                        const double* __restrict__ Li, const int* __restrict__ Ps)
+{
+    __shared__ double S[2][QRB][QRB + 1];   // [0]: A12, [1]: U12
+    typedef Mfma<double>::acc_t acc_t;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lr = lane >> 4, lc = lane & 15;
+    const int R = N - p0, cols = W - (p0 + nb), nrt = (R + 15) >> 4;
+    int b, cb;
+    chain_tile((cols + QRB - 1) / QRB, batch, b, cb);
+    const int c0 = cb * QRB;
+    double* A = bufs + (long long)b * stride;
+    const double* L = Ls + (long long)b * N * QRB;
+    const double* Lv = Li + (long long)b * QRB * QRB;
+    const int* P = Ps + (long long)b * N;
+    const long long base = (long long)p0 * W + p0 + nb + c0;   // At(0, 0)
+
+    acc_t at[4][2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int row = 16 * (wave + 4 * q) + 4 * r + lr;
+            const int src = row < R ? P[row] : 0;
+#pragma unroll
+            for (int ct = 0; ct < 2; ++ct) {
+                const int col = 16 * ct + lc;
+                at[q][ct][r] = (row < R && c0 + col < cols) ? A[base + (long long)src * W + col] : 0.0;
+            }
+        }
+    if (wave < 2) {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) S[0][16 * wave + 4 * r + lr][16 * ct + lc] = at[0][ct][r];
+    }
+    __syncthreads();
+    if (wave < 2) {
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+            for (int k0 = 0; k0 < QRB; k0 += 4)
+                acc = Mfma<double>::mma(Lv[(16 * wave + lc) + (long long)(k0 + lr) * QRB], S[0][k0 + lr][16 * ct + lc], acc);
+            at[0][ct] = acc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) S[1][16 * wave + 4 * r + lr][16 * ct + lc] = acc[r];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int rt = wave + 4 * q;
+        if (rt < 2 || rt >= nrt) continue;
+        const int arow = 16 * rt + lc;
+        double av[QRB / 4];
+#pragma unroll
+        for (int s = 0; s < QRB / 4; ++s) av[s] = (arow < R) ? -L[arow + (long long)(4 * s + lr) * N] : 0.0;
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+            acc_t acc = at[q][ct];
+#pragma unroll
+            for (int s = 0; s < QRB / 4; ++s) acc = Mfma<double>::mma(av[s], S[1][4 * s + lr][16 * ct + lc], acc);
+            at[q][ct] = acc;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (wave + 4 * q >= nrt) continue;
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = 16 * (wave + 4 * q) + 4 * r + lr, col = 16 * ct + lc;
+                if (row < R && c0 + col < cols) A[base + (long long)row * W + col] = at[q][ct][r];
+            }
+    }
+}
+
+hipError_t launch_lu_blocked(double* buf, long long stride, int N, int W, int batch, double* Ls, double* Li, int* Ps,
+                             uint32_t* health, hipStream_t s)
+{
+    if (N > 256 || N < 1 || W <= N) return hipErrorInvalidValue;
+    for (int p0 = 0; p0 < N; p0 += QRB) {
+        const int nb = (N - p0 < QRB) ? N - p0 : QRB;
+        const int cols = W - (p0 + nb);
+        hipLaunchKernelGGL(lu_panel_kernel, dim3(batch), dim3(256), 0, s, buf, stride, N, W, p0, nb, Ls, Li, Ps, health);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(lu_update_kernel, dim3(((cols + QRB - 1) / QRB) * batch), dim3(256), 0, s, buf, stride, N,
+                           W, batch, p0, nb, Ls, Li, Ps);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------------------------
 //  Blocked back-substitution X = R^{-1} Z for the row-major system (R in columns [0, N), Z in
 //  columns [c0, c0+P), solved in place), 32-row blocks from the bottom:
 //    trsm_diag_kernel  X_k = R_kk^{-1} Z_k, one thread per right-hand side, R_kk in LDS;
