@@ -116,6 +116,7 @@ struct dqmc_ctx {
     long long eq_count = 0, dyn_count = 0;
     double *meas_cache = nullptr, *meas_data = nullptr, *meas_part = nullptr, *costab = nullptr;
     double* wd_bound = nullptr;      // watchdog bound on max|G_recomputed - G_working| (0 = off)
+    double* cmp_part = nullptr;      // [B][16][3] partial reductions of compare_commit
     DStack left, right;
     // parameter adjoint at fixed fields (allocated on first use)
     int* nb = nullptr;
@@ -524,7 +525,7 @@ int stabilize_eq(dqmc_ctx* c)
 {
     RCHECK(recompute_eq(c, c->W3));
     HCHECK(c, launch_compare_commit(c->W3, c->G64, c->prec == DQMC_MIXED ? c->G32 : nullptr, c->NN(), c->N, c->wrap_eq,
-                                    c->maxdG, c->wd_bound, c->health, c->B, c->stream));
+                                    c->maxdG, c->wd_bound, c->health, c->cmp_part, c->B, c->stream));
     return set_identity_stk(c);
 }
 
@@ -591,7 +592,7 @@ int do_sweep_dynamic(dqmc_ctx* c)
             RCHECK(stack_push(c, c->left));
             RCHECK(recompute_dyn(c, c->W3, c->W2, c->W1));
             HCHECK(c, launch_compare_commit(c->W3, c->G64, c->prec == DQMC_MIXED ? c->G32 : nullptr, nn, N, c->wrap_eq,
-                                            c->maxdG, c->wd_bound, c->health, c->B, c->stream));
+                                            c->maxdG, c->wd_bound, c->health, c->cmp_part, c->B, c->stream));
             HCHECK(c, launch_compare_only(c->W2, c->gt0, nn, N, c->wrap_dyn, c->health, c->B, c->stream));
             HCHECK(c, launch_compare_only(c->W1, c->g0t, nn, N, c->wrap_dyn, c->health, c->B, c->stream));
             HCHECK(c, launch_copy_d(c->W2, nn, c->gt0, nn, nn, c->B, c->stream));
@@ -886,6 +887,7 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     RCHECK(c->alloc(&c->wrap_eq, (size_t)B * 8)); RCHECK(c->alloc(&c->wrap_dyn, (size_t)B * 8));
     RCHECK(c->alloc(&c->maxdG, (size_t)B * 8));
     RCHECK(c->alloc(&c->health, (size_t)B * 4)); RCHECK(c->alloc(&c->wd_bound, 8));
+    RCHECK(c->alloc(&c->cmp_part, (size_t)B * 16 * 3 * 8));
 
     // constant tables
     std::vector<double> ek, iek;

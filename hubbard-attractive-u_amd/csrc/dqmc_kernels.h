@@ -166,9 +166,10 @@ hipError_t launch_extract(const double* buf, long long sbuf, int W, int c0, doub
 // non-null).  err_signed / err_abs are per-chain running maxima (updated in place; the caller
 // starts them at 0 like src/dqmc_core.cpp:211-212).  Sets HEALTH_NONFINITE_G for a non-finite
 // entry on either side and HEALTH_WATCHDOG when *bound > 0 and the max-abs difference exceeds it.
+// part: per-chain scratch of 16 x 3 doubles for the two-stage reduction.
 hipError_t launch_compare_commit(const double* Gnew, double* G64, float* G32, long long sG, int N,
                                  double* err_signed, double* err_abs, const double* bound, uint32_t* health,
-                                 int batch, hipStream_t s);
+                                 double* part, int batch, hipStream_t s);
 // Same but the working matrix is FP64 only (dynamic G(t,0), G(0,t)); no commit.
 hipError_t launch_compare_only(const double* Gnew, const double* Gcur, long long sG, int N, double* err_signed,
                                uint32_t* health, int batch, hipStream_t s);

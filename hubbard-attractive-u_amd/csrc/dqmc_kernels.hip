@@ -2042,23 +2042,35 @@ hipError_t launch_trsm_upper_v1(double* buf, long long stride, int N, int W, int
 // ---------------------------------------------------------------------------------------------
 //  UDV helpers
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void udv_prepare_kernel(const double* __restrict__ Ms, long long sM,
-                                                          double* __restrict__ bufs, long long sbuf,
-                                                          int* __restrict__ perms, int N)
+// Column norms of M (column-major, ld N) and the descending-norm order (ties to the lower index,
+// a stable sort): one workgroup per chain, wave w reduces the column groups 8 w .. 8 w + 7,
+// 8 w + 32 .., with its lanes reading consecutive rows.
+__global__ __launch_bounds__(256) void udv_rank_kernel(const double* __restrict__ Ms, long long sM,
+                                                       int* __restrict__ perms, int N)
 {
     __shared__ double nrm[1024];
-    const int b = blockIdx.x, tid = threadIdx.x;
+    const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double* M = Ms + (long long)b * sM;
-    double* buf = bufs + (long long)b * sbuf;
     int* perm = perms + (long long)b * N;
-    const int W = 2 * N;
-    for (int j = tid; j < N; j += 256) {
-        double s = 0.0;
-        for (int r = 0; r < N; ++r) { const double v = M[r + (long long)j * N]; s += v * v; }
-        nrm[j] = s;
+    // 8 columns per step, so that 8 x ceil(N / 64) loads per lane are in flight together
+    for (int j0 = wave * 8; j0 < N; j0 += 32) {
+        double s[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) s[c] = 0.0;
+        for (int r = lane; r < N; r += 64) {
+            double v[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) v[c] = (j0 + c < N) ? M[r + (long long)(j0 + c) * N] : 0.0;
+#pragma unroll
+            for (int c = 0; c < 8; ++c) s[c] += v[c] * v[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const double t = wave_sum_dpp(s[c]);
+            if (lane == 0 && j0 + c < N) nrm[j0 + c] = t;
+        }
     }
     __syncthreads();
-    // rank by descending norm, ties broken by index (a stable sort)
     for (int j = tid; j < N; j += 256) {
         const double nj = nrm[j];
         int rank = 0;
@@ -2068,11 +2080,34 @@ __global__ __launch_bounds__(256) void udv_prepare_kernel(const double* __restri
         }
         perm[rank] = j;
     }
+}
+
+// buf[r][j] = M(r, perm[j]) and buf[r][N + j] = delta_rj, through a 32 x 32 LDS tile so that the
+// column reads of M and the row writes of buf are both contiguous; grid (tiles, chains).
+__global__ __launch_bounds__(256) void udv_copy_kernel(const double* __restrict__ Ms, long long sM,
+                                                       double* __restrict__ bufs, long long sbuf,
+                                                       const int* __restrict__ perms, int N)
+{
+    __shared__ double t[32][33];
+    const int b = blockIdx.y, nt = (N + 31) / 32;
+    const int r0 = (blockIdx.x % nt) * 32, j0 = (blockIdx.x / nt) * 32;
+    const double* M = Ms + (long long)b * sM;
+    double* buf = bufs + (long long)b * sbuf;
+    const int* perm = perms + (long long)b * N;
+    const int W = 2 * N;
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        const int rr = e & 31, jj = e >> 5;
+        const int r = r0 + rr, j = j0 + jj;
+        t[jj][rr] = (r < N && j < N) ? M[r + (long long)perm[j] * N] : 0.0;
+    }
     __syncthreads();
-    for (int e = tid; e < N * N; e += 256) {
-        const int r = e / N, j = e % N;
-        buf[(long long)r * W + j] = M[r + (long long)perm[j] * N];
-        buf[(long long)r * W + N + j] = (r == j) ? 1.0 : 0.0;
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        const int jj = e & 31, rr = e >> 5;
+        const int r = r0 + rr, j = j0 + jj;
+        if (r < N && j < N) {
+            buf[(long long)r * W + j] = t[jj][rr];
+            buf[(long long)r * W + N + j] = (r == j) ? 1.0 : 0.0;
+        }
     }
 }
 
@@ -2080,7 +2115,11 @@ hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long l
                               int batch, hipStream_t s)
 {
     if (N > 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(udv_prepare_kernel, dim3(batch), dim3(256), 0, s, M, sM, buf, sbuf, perm, N);
+    hipLaunchKernelGGL(udv_rank_kernel, dim3(batch), dim3(256), 0, s, M, sM, perm, N);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int nt = (N + 31) / 32;
+    hipLaunchKernelGGL(udv_copy_kernel, dim3(nt * nt, batch), dim3(256), 0, s, M, sM, buf, sbuf, perm, N);
     return hipGetLastError();
 }
 
@@ -2205,19 +2244,24 @@ __device__ __forceinline__ double block_max(double v, double* red)
     return r;
 }
 
+// Stage 1: workgroup p of chain b takes entries [p * span, (p + 1) * span) of the N x N matrices,
+// commits them and writes its partial (signed max, max-abs, non-finite flag) to part[b][p].
+// Stage 2: one thread per chain folds the parts in order into the running maxima and the health
+// flags (no atomics: the same bits on every run).
+#define CMP_PARTS 16
 __global__ __launch_bounds__(256) void compare_commit_kernel(const double* __restrict__ Gnew, double* __restrict__ G64,
-                                                             float* __restrict__ G32, long long sG, int N,
-                                                             double* err_signed, double* err_abs,
-                                                             const double* __restrict__ bound, uint32_t* health)
+                                                             float* __restrict__ G32, long long sG, int N, int span,
+                                                             double* __restrict__ part)
 {
     __shared__ double red[4];
-    const int b = blockIdx.x;
+    const int p = blockIdx.x, b = blockIdx.y, np = gridDim.x;
     const double* gn = Gnew + (long long)b * sG;
     double* g64 = G64 + (long long)b * sG;
     float* g32 = G32 ? G32 + (long long)b * sG : nullptr;
+    const long long e0 = (long long)p * span, e1 = min((long long)N * N, e0 + span);
     double es = -DBL_MAX, ea = 0.0;
     int bad = 0;
-    for (long long e = threadIdx.x; e < (long long)N * N; e += 256) {
+    for (long long e = e0 + threadIdx.x; e < e1; e += 256) {
         const double v = gn[e];
         const double cur = g32 ? (double)g32[e] : g64[e];
         const double d = v - cur;
@@ -2231,22 +2275,45 @@ __global__ __launch_bounds__(256) void compare_commit_kernel(const double* __res
     ea = block_max(ea, red);
     const double nanflag = block_max(bad ? 1.0 : 0.0, red);
     if (threadIdx.x == 0) {
-        if (es > err_signed[b]) err_signed[b] = es;
-        if (ea > err_abs[b] || nanflag > 0.0) err_abs[b] = nanflag > 0.0 ? INFINITY : fmax(err_abs[b], ea);
-        uint32_t h = health[b];
-        if (nanflag > 0.0) h |= HEALTH_NONFINITE_G;
-        const double lim = *bound;
-        if (lim > 0.0 && ea > lim) h |= HEALTH_WATCHDOG;
-        health[b] = h;
+        double* q = part + ((long long)b * np + p) * 3;
+        q[0] = es; q[1] = ea; q[2] = nanflag;
     }
+}
+
+__global__ void compare_fold_kernel(const double* __restrict__ part, int np, double* err_signed, double* err_abs,
+                                    const double* __restrict__ bound, uint32_t* health, int batch)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= batch) return;
+    double es = -DBL_MAX, ea = 0.0, nanflag = 0.0;
+    for (int p = 0; p < np; ++p) {
+        const double* q = part + ((long long)b * np + p) * 3;
+        es = fmax(es, q[0]);
+        ea = fmax(ea, q[1]);
+        nanflag = fmax(nanflag, q[2]);
+    }
+    if (es > err_signed[b]) err_signed[b] = es;
+    if (ea > err_abs[b] || nanflag > 0.0) err_abs[b] = nanflag > 0.0 ? INFINITY : fmax(err_abs[b], ea);
+    uint32_t h = health[b];
+    if (nanflag > 0.0) h |= HEALTH_NONFINITE_G;
+    const double lim = *bound;
+    if (lim > 0.0 && ea > lim) h |= HEALTH_WATCHDOG;
+    health[b] = h;
 }
 
 hipError_t launch_compare_commit(const double* Gnew, double* G64, float* G32, long long sG, int N,
                                  double* err_signed, double* err_abs, const double* bound, uint32_t* health,
-                                 int batch, hipStream_t s)
+                                 double* part, int batch, hipStream_t s)
 {
-    hipLaunchKernelGGL(compare_commit_kernel, dim3(batch), dim3(256), 0, s, Gnew, G64, G32, sG, N, err_signed,
-                       err_abs, bound, health);
+    const long long nn = (long long)N * N;
+    int np = (int)((nn + 4095) / 4096);
+    if (np > CMP_PARTS) np = CMP_PARTS;
+    const int span = (int)((nn + np - 1) / np);
+    hipLaunchKernelGGL(compare_commit_kernel, dim3(np, batch), dim3(256), 0, s, Gnew, G64, G32, sG, N, span, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(compare_fold_kernel, dim3((batch + 63) / 64), dim3(64), 0, s, part, np, err_signed, err_abs,
+                       bound, health, batch);
     return hipGetLastError();
 }
 
