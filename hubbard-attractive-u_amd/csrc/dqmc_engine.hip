@@ -125,7 +125,11 @@ struct dqmc_ctx {
            *adj_out = nullptr;
 
     long long NN() const { return (long long)N * N; }
-    long long sqr() const { return (long long)N * 3 * N; }
+    // Row-major recompose / UDV systems [A | rhs] are N rows of weq() (equal-time: N + N) or wdyn()
+    // (time-displaced: N + 2N) doubles.
+    int weq() const { return 2 * N; }
+    int wdyn() const { return 3 * N; }
+    long long sqr() const { return (long long)N * wdyn(); }
 
     int fail(int code, const char* fmt, ...)
     {
@@ -260,7 +264,7 @@ int gemm(dqmc_ctx* c, const T* A, long long sA, int lda, int tA, const T* B, lon
 hipError_t qr_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int batch, hipStream_t s)
 {
     if (c->qr_mode == 1) return launch_qr_v1(buf, stride, N, W, batch, s);
-    return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrT, c->qrWt, 3LL * N * 32, c->tile64, c->qr_update,
+    return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrT, c->qrWt, (long long)c->wdyn() * 32, c->tile64, c->qr_update,
                              c->qr_panel, s);
 }
 hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int c0, int P, int batch, hipStream_t s)
@@ -411,9 +415,9 @@ int stack_push(dqmc_ctx* c, DStack& s)
                             dvec(DIAG_VEC, sD(c, s, s.len - 1), sd)));
         M = c->W1;
     }
-    HCHECK(c, launch_udv_prepare(M, nn, c->qr, c->sqr(), c->perm, N, c->B, c->stream));
-    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
-    HCHECK(c, launch_udv_finish(c->qr, c->sqr(), c->perm, sU(c, s, s.len), st, sD(c, s, s.len), sd, c->W2, nn, N,
+    HCHECK(c, launch_udv_prepare(M, nn, c->qr, c->sqr(), c->weq(), c->perm, N, c->B, c->stream));
+    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, c->weq(), c->B, c->stream));
+    HCHECK(c, launch_udv_finish(c->qr, c->sqr(), c->weq(), c->perm, sU(c, s, s.len), st, sD(c, s, s.len), sd, c->W2, nn, N,
                                 c->health, c->B, c->stream));
     if (s.len == 0)
         HCHECK(c, launch_copy_d(c->W2, nn, sV(c, s, 0), st, nn, c->B, c->stream));
@@ -429,10 +433,11 @@ int solve_00bb(dqmc_ctx* c, const double* u, const double* v, const double* d, d
 {
     const long long nn = c->NN(), st = (long long)c->S * nn, sd = (long long)c->S * c->N;
     const int N = c->N;
-    HCHECK(c, launch_build_00bb(u, st, v, st, d, sd, c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
-    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, 2 * N, c->B, c->stream));
-    TIMED(c, KT_TRSM, trsm_any(c, c->qr, c->sqr(), N, 2 * N, N, N, c->B, c->stream));
-    HCHECK(c, launch_extract(c->qr, c->sqr(), 2 * N, N, out, nn, N, transposed_result ? 0 : 1, c->B, c->stream));
+    const int W = c->weq();
+    HCHECK(c, launch_build_00bb(u, st, v, st, d, sd, c->qr, c->sqr(), N, W, c->B, c->stream));
+    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, W, c->B, c->stream));
+    TIMED(c, KT_TRSM, trsm_any(c, c->qr, c->sqr(), N, W, N, N, c->B, c->stream));
+    HCHECK(c, launch_extract(c->qr, c->sqr(), W, N, out, nn, N, transposed_result ? 0 : 1, c->B, c->stream));
     return 0;
 }
 
@@ -440,7 +445,7 @@ int solve_00bb(dqmc_ctx* c, const double* u, const double* v, const double* d, d
 int recompute_eq(dqmc_ctx* c, double* out)
 {
     const long long nn = c->NN(), st = (long long)c->S * nn, sd = (long long)c->S * c->N;
-    const int N = c->N, W = 2 * N;
+    const int N = c->N, W = c->weq();
     DStack &Ls = c->left, &Rs = c->right;
     if (Ls.len == 0) {
         // G^T = (1 + Ur Dr Vr^T)^{-1}
@@ -482,7 +487,7 @@ int recompute_dyn(dqmc_ctx* c, double* gtt, double* gt0n, double* g0tn)
     }
     const double *Ul = sU(c, Ls, Ls.len - 1), *Vl = sV(c, Ls, Ls.len - 1), *Dl = sD(c, Ls, Ls.len - 1);
     const double *Ur = sU(c, Rs, Rs.len - 1), *Vr = sV(c, Rs, Rs.len - 1), *Dr = sD(c, Rs, Rs.len - 1);
-    const int W = 3 * N;
+    const int W = c->wdyn();
     RCHECK(gemm<double>(c, Ul, st, N, 1, Ur, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_INVMAX1, Dl, sd), dnone(),
                         dvec(DIAG_INVMAX1, Dr, sd)));
     RCHECK(gemm<double>(c, Vl, st, N, 1, Vr, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_MIN1, Dl, sd), dnone(),
@@ -872,7 +877,7 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     RCHECK(c->alloc(&c->perm, (size_t)B * N * 4));
     RCHECK(c->alloc(&c->lu_perm, (size_t)B * N * 4));
     RCHECK(c->alloc(&c->qrV, (size_t)B * N * 32 * 8)); RCHECK(c->alloc(&c->qrT, (size_t)B * 32 * 32 * 8));
-    RCHECK(c->alloc(&c->qrWt, (size_t)2 * B * 3 * N * 32 * 8));
+    RCHECK(c->alloc(&c->qrWt, (size_t)2 * B * c->wdyn() * 32 * 8));
     for (DStack* s : {&c->left, &c->right}) {
         RCHECK(c->alloc(&s->U, (size_t)B * S * nn * 8));
         RCHECK(c->alloc(&s->V, (size_t)B * S * nn * 8));
@@ -1046,7 +1051,11 @@ int dqmc_init(dqmc_ctx* c)
     HCHECK(c, launch_fill_d(c->wrap_eq, B, 0.0, c->stream));
     HCHECK(c, launch_fill_d(c->wrap_dyn, B, 0.0, c->stream));
     HCHECK(c, launch_fill_d(c->maxdG, B, 0.0, c->stream));
-    if (!c->initialized) RCHECK(autotune(c));
+    if (!c->initialized) {
+        RCHECK(autotune(c));
+        const char* gt = std::getenv("DQMC_GEMM_TILE");   // override of the autotuner's choice
+        if (gt && gt[0] >= '0' && gt[0] < '0' + TILE_COUNT && gt[1] == 0) c->tile64 = c->tile32 = gt[0] - '0';
+    }
     RCHECK(health_reset(c));
     RCHECK(build_from_fields(c));
     HCHECK(c, hipStreamSynchronize(c->stream));
@@ -1330,7 +1339,7 @@ int dqmc_forward_fixed(dqmc_ctx* c, double* logdet, double* gdiag)
     RCHECK(save_stats(c, st));
     RCHECK(health_reset(c));
     RCHECK(build_from_fields(c));
-    HCHECK(c, launch_logdet_00bb(c->qr, c->sqr(), 2 * N, sD(c, c->right, c->right.len - 1), (long long)c->S * N, N,
+    HCHECK(c, launch_logdet_00bb(c->qr, c->sqr(), c->weq(), sD(c, c->right, c->right.len - 1), (long long)c->S * N, N,
                                  c->adj_out, B, c->stream));
     RCHECK(do_sweep_dynamic(c));
     HCHECK(c, launch_gf_diag(c->vecgftt, (long long)L * nn, L, N, c->adj_gdiag, B, c->stream));

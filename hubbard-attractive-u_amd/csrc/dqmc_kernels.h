@@ -76,7 +76,8 @@ template <typename T>
 hipError_t launch_kron_right(const KronArgs<T>& a, hipStream_t s);
 
 // Tile shapes the autotuner chooses from.
-enum GemmTile : int { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_COUNT = 3 };
+// TILE_64x64_PF: 64 x 64 with double-buffered LDS and global loads two K steps ahead.
+enum GemmTile : int { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_64x64_PF = 3, TILE_COUNT = 4 };
 
 template <typename T>
 hipError_t launch_gemm(const GemmArgs<T>& a, int tile, hipStream_t s);
@@ -137,12 +138,12 @@ hipError_t launch_trsm_upper_v1(double* buf, long long stride, int N, int W, int
                                 hipStream_t s);
 
 // Pre-pivoted UDV input: M column-major (ld N); writes buf[r][j] = M(r, perm[j]) with perm
-// sorting the columns of M by descending 2-norm, and buf[r][N+j] = delta_rj.
-hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long long sbuf, int* perm,
+// sorting the columns of M by descending 2-norm, and buf[r][N+j] = delta_rj (row length W >= 2N).
+hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long long sbuf, int W, int* perm,
                               int N, int batch, hipStream_t s);
 // After QR: D_i = |R_ii|, Tt (column-major N x N) with Tt(perm[j], i) = R_ij / D_i (j >= i),
 // U (column-major) = Q read from the trailing block.
-hipError_t launch_udv_finish(const double* buf, long long sbuf, const int* perm, double* U, long long sU,
+hipError_t launch_udv_finish(const double* buf, long long sbuf, int W, const int* perm, double* U, long long sU,
                              double* D, long long sD, double* Tt, long long sTt, int N, uint32_t* health, int batch,
                              hipStream_t s);
 

@@ -202,6 +202,139 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> g)
             }
 }
 
+// 64 x 64 output tile with compile-time operand layouts, LDS tiles double-buffered (one barrier
+// per K step) and the global loads two K steps ahead in two register stages, so a load has the
+// MFMA work of two steps to arrive.  The loop runs two steps per iteration so that every register
+// stage is addressed statically.
+template <typename T, bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs<T> g)
+{
+    constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32, NA = BM * BK / 256, NB = BN * BK / 256;
+    constexpr int PAD = (sizeof(T) == 8) ? 1 : 2;
+    __shared__ T As[2][BK][BM + PAD];
+    __shared__ T Bs[2][BK][BN + PAD];
+    __shared__ double sk[256], srow[BM], scol[BN];   // diagonal scalings, evaluated once per tile
+    typedef typename Mfma<T>::acc_t acc_t;
+
+    const int gm = (g.M + BM - 1) / BM, gn = (g.Nc + BN - 1) / BN;
+    int b, t;
+    chain_tile(gm * gn, g.batch, b, t);
+    const int m0 = (t % gm) * BM, n0 = (t / gm) * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const T* __restrict__ A = g.A + (long long)b * g.sA;
+    const T* __restrict__ B = g.B + (long long)b * g.sB;
+    T* __restrict__ C = g.C + (long long)b * g.sC;
+    const bool innd = g.innd.mode != DIAG_NONE;
+    if (innd)
+        for (int kk = tid; kk < g.K && kk < 256; kk += 256) sk[kk] = diag_value(g.innd, b, kk);
+    if (tid < BM) srow[tid] = (g.rowd.mode != DIAG_NONE && m0 + tid < g.M) ? diag_value(g.rowd, b, m0 + tid) : 1.0;
+    else if (tid < BM + BN)
+        scol[tid - BM] = (g.cold.mode != DIAG_NONE && n0 + tid - BM < g.Nc) ? diag_value(g.cold, b, n0 + tid - BM) : 1.0;
+    __syncthreads();
+
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    T ra0[NA], rb0[NB], ra1[NA], rb1[NB];
+    auto fetch = [&](int k0, T (&ra)[NA], T (&rb)[NB]) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = u * 256 + tid;
+            const int i = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
+            const int gi = m0 + i, gk = k0 + kk;
+            T v = T(0);
+            if (gi < g.M && gk < g.K) {
+                v = TA ? A[gk + (long long)gi * g.lda] : A[gi + (long long)gk * g.lda];
+                if (innd) v = T((double)v * sk[gk]);
+            }
+            ra[u] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = u * 256 + tid;
+            const int j = TB ? e % BN : e / BK, kk = TB ? e / BN : e % BK;
+            const int gj = n0 + j, gk = k0 + kk;
+            rb[u] = (gj < g.Nc && gk < g.K) ? (TB ? B[gj + (long long)gk * g.ldb] : B[gk + (long long)gj * g.ldb]) : T(0);
+        }
+    };
+    auto stash = [&](int buf, const T (&ra)[NA], const T (&rb)[NB]) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = u * 256 + tid;
+            As[buf][TA ? e % BK : e / BM][TA ? e / BK : e % BM] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = u * 256 + tid;
+            Bs[buf][TB ? e / BN : e % BK][TB ? e % BN : e / BK] = rb[u];
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T af[2], bf[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) af[i] = As[buf][kk + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bf[j] = Bs[buf][kk + (lane >> 4)][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
+        }
+    };
+
+    const int nk = (g.K + BK - 1) / BK;
+    fetch(0, ra0, rb0);
+    if (nk > 1) fetch(BK, ra1, rb1);
+    stash(0, ra0, rb0);
+    __syncthreads();
+    // invariant at the top of step ks: LDS buffer ks&1 holds step ks, register stage (ks+1)&1 holds
+    // step ks+1 (if any), register stage ks&1 is free
+    for (int ks = 0; ks < nk; ks += 2) {
+        if (ks + 2 < nk) fetch((ks + 2) * BK, ra0, rb0);
+        compute(0);
+        if (ks + 1 < nk) stash(1, ra1, rb1);
+        __syncthreads();
+        if (ks + 1 >= nk) break;
+        if (ks + 3 < nk) fetch((ks + 3) * BK, ra1, rb1);
+        compute(1);
+        if (ks + 2 < nk) stash(0, ra0, rb0);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm * WM + i * 16 + Mfma<T>::row(lane, r);
+                const int gj = n0 + wn * WN + j * 16 + (lane & 15);
+                if (gi < g.M && gj < g.Nc) {
+                    double v = (double)acc[i][j][r] * g.alpha;
+                    if (g.rowd.mode != DIAG_NONE) v *= srow[gi - m0];
+                    if (g.cold.mode != DIAG_NONE) v *= scol[gj - n0];
+                    const long long off = g.tC ? (gj + (long long)gi * g.ldc) : (gi + (long long)gj * g.ldc);
+                    if (g.beta != 0.0) v += g.beta * (double)C[off];
+                    C[off] = T(v);
+                }
+            }
+}
+
+template <typename T, bool TA, bool TB>
+static hipError_t launch_gemm64(const GemmArgs<T>& a, hipStream_t s)
+{
+    const long long blocks = (long long)((a.M + 63) / 64) * ((a.Nc + 63) / 64) * a.batch;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm64_kernel<T, TA, TB>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 template <typename T, int BM, int BN>
 static hipError_t launch_gemm_tile(const GemmArgs<T>& a, hipStream_t s)
 {
@@ -218,6 +351,10 @@ hipError_t launch_gemm(const GemmArgs<T>& a, int tile, hipStream_t s)
     switch (tile) {
         case TILE_32x32: return launch_gemm_tile<T, 32, 32>(a, s);
         case TILE_64x32: return launch_gemm_tile<T, 64, 32>(a, s);
+        case TILE_64x64_PF:
+            if (a.innd.mode != DIAG_NONE && a.K > 256) return launch_gemm_tile<T, 64, 64>(a, s);
+            if (a.tA) return a.tB ? launch_gemm64<T, true, true>(a, s) : launch_gemm64<T, true, false>(a, s);
+            return a.tB ? launch_gemm64<T, false, true>(a, s) : launch_gemm64<T, false, false>(a, s);
         default: return launch_gemm_tile<T, 64, 64>(a, s);
     }
 }
@@ -2086,7 +2223,7 @@ __global__ __launch_bounds__(256) void udv_rank_kernel(const double* __restrict_
 // column reads of M and the row writes of buf are both contiguous; grid (tiles, chains).
 __global__ __launch_bounds__(256) void udv_copy_kernel(const double* __restrict__ Ms, long long sM,
                                                        double* __restrict__ bufs, long long sbuf,
-                                                       const int* __restrict__ perms, int N)
+                                                       const int* __restrict__ perms, int N, int W)
 {
     __shared__ double t[32][33];
     const int b = blockIdx.y, nt = (N + 31) / 32;
@@ -2094,7 +2231,6 @@ __global__ __launch_bounds__(256) void udv_copy_kernel(const double* __restrict_
     const double* M = Ms + (long long)b * sM;
     double* buf = bufs + (long long)b * sbuf;
     const int* perm = perms + (long long)b * N;
-    const int W = 2 * N;
     for (int e = threadIdx.x; e < 1024; e += 256) {
         const int rr = e & 31, jj = e >> 5;
         const int r = r0 + rr, j = j0 + jj;
@@ -2111,7 +2247,7 @@ __global__ __launch_bounds__(256) void udv_copy_kernel(const double* __restrict_
     }
 }
 
-hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long long sbuf, int* perm, int N,
+hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long long sbuf, int W, int* perm, int N,
                               int batch, hipStream_t s)
 {
     if (N > 1024) return hipErrorInvalidValue;
@@ -2119,14 +2255,14 @@ hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long l
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int nt = (N + 31) / 32;
-    hipLaunchKernelGGL(udv_copy_kernel, dim3(nt * nt, batch), dim3(256), 0, s, M, sM, buf, sbuf, perm, N);
+    hipLaunchKernelGGL(udv_copy_kernel, dim3(nt * nt, batch), dim3(256), 0, s, M, sM, buf, sbuf, perm, N, W);
     return hipGetLastError();
 }
 
 __global__ __launch_bounds__(256) void udv_finish_kernel(const double* __restrict__ bufs, long long sbuf,
                                                          const int* __restrict__ perms, double* __restrict__ Us,
                                                          long long sU, double* __restrict__ Ds, long long sD,
-                                                         double* __restrict__ Tts, long long sTt, int N,
+                                                         double* __restrict__ Tts, long long sTt, int N, int W,
                                                          uint32_t* health)
 {
     const int b = blockIdx.y;
@@ -2135,7 +2271,6 @@ __global__ __launch_bounds__(256) void udv_finish_kernel(const double* __restric
     double* U = Us + (long long)b * sU;
     double* D = Ds + (long long)b * sD;
     double* Tt = Tts + (long long)b * sTt;
-    const int W = 2 * N;
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= (long long)N * N) return;
     const int i = (int)(e / N), j = (int)(e % N);
@@ -2151,12 +2286,12 @@ __global__ __launch_bounds__(256) void udv_finish_kernel(const double* __restric
     Tt[perm[j] + (long long)i * N] = (j >= i) ? buf[(long long)i * W + j] / d : 0.0;
 }
 
-hipError_t launch_udv_finish(const double* buf, long long sbuf, const int* perm, double* U, long long sU,
+hipError_t launch_udv_finish(const double* buf, long long sbuf, int W, const int* perm, double* U, long long sU,
                              double* D, long long sD, double* Tt, long long sTt, int N, uint32_t* health, int batch,
                              hipStream_t s)
 {
     dim3 grid((unsigned)(((long long)N * N + 255) / 256), batch);
-    hipLaunchKernelGGL(udv_finish_kernel, grid, dim3(256), 0, s, buf, sbuf, perm, U, sU, D, sD, Tt, sTt, N, health);
+    hipLaunchKernelGGL(udv_finish_kernel, grid, dim3(256), 0, s, buf, sbuf, perm, U, sU, D, sD, Tt, sTt, N, W, health);
     return hipGetLastError();
 }
 

@@ -297,11 +297,11 @@ class Engine:
     def tuning(self):
         """Autotuned GEMM tiles {'fp64': name, 'fp32': name, 'ms': {(prec, name): ms}}."""
         a, b = C.c_int(), C.c_int()
-        ms = (C.c_float * 6)()
+        ms = (C.c_float * 8)()
         self._chk(lib().dqmc_get_tuning(self._h, C.byref(a), C.byref(b), ms), "get_tuning")
-        names = ["64x64", "32x32", "64x32"]
+        names = ["64x64", "32x32", "64x32", "64x64pf"]
         return {"fp64": names[a.value], "fp32": names[b.value],
-                "ms": {(p, names[t]): ms[i * 3 + t] for i, p in enumerate(("fp64", "fp32")) for t in range(3)}}
+                "ms": {(p, names[t]): ms[i * 4 + t] for i, p in enumerate(("fp64", "fp32")) for t in range(4)}}
 
     def forward_fixed(self):
         """(logdet [chains], gdiag [chains, nt, ns]) at the current fields, no updates."""
@@ -326,7 +326,7 @@ class Engine:
         return out
 
     def set_tuning(self, fp64, fp32):
-        names = ["64x64", "32x32", "64x32"]
+        names = ["64x64", "32x32", "64x32", "64x64pf"]
         self._chk(lib().dqmc_set_tuning(self._h, names.index(fp64), names.index(fp32)), "set_tuning")
 
     def event_elapsed(self, a, b):

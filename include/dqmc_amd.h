@@ -159,8 +159,9 @@ int dqmc_kernel_timer_end(dqmc_ctx* ctx, double* total_ms, int* launches);
 int dqmc_use_graphs(dqmc_ctx* ctx, int on);
 
 /* GEMM tile chosen by the autotuner in dqmc_init for FP64 (and FP32) products:
- * 0 = 64x64, 1 = 32x32, 2 = 64x32 output tiles; ms (optional, 2 x 3 floats) receives the
- * measured time per wrap-shaped product of every candidate.  No counterpart in the reference. */
+ * 0 = 64x64, 1 = 32x32, 2 = 64x32 output tiles, 3 = 64x64 with double-buffered LDS and global
+ * loads two steps ahead; ms (optional, 2 x 4 floats) receives the measured time per wrap-shaped
+ * product of every candidate.  No counterpart in the reference. */
 int dqmc_get_tuning(dqmc_ctx* ctx, int* tile_fp64, int* tile_fp32, float* ms);
 /* Override the autotuner's choice (after dqmc_init, before the first captured sweep). */
 int dqmc_set_tuning(dqmc_ctx* ctx, int tile_fp64, int tile_fp32);
