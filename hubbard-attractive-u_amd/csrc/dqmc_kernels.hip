@@ -247,11 +247,8 @@ __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs<T> g)
             const int i = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
             const int gi = m0 + i, gk = k0 + kk;
             T v = T(0);
-            if (gi < g.M && gk < g.K) {
-                v = TA ? A[gk + (long long)gi * g.lda] : A[gi + (long long)gk * g.lda];
-                if (innd) v = T((double)v * sk[gk]);
-            }
-            ra[u] = v;
+            if (gi < g.M && gk < g.K) v = TA ? A[gk + (long long)gi * g.lda] : A[gi + (long long)gk * g.lda];
+            ra[u] = v;   // the inner scaling is applied at the LDS store, so the load is not waited on here
         }
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
@@ -261,11 +258,14 @@ __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs<T> g)
             rb[u] = (gj < g.Nc && gk < g.K) ? (TB ? B[gj + (long long)gk * g.ldb] : B[gk + (long long)gj * g.ldb]) : T(0);
         }
     };
-    auto stash = [&](int buf, const T (&ra)[NA], const T (&rb)[NB]) {
+    auto stash = [&](int buf, int k0, const T (&ra)[NA], const T (&rb)[NB]) {
 #pragma unroll
         for (int u = 0; u < NA; ++u) {
             const int e = u * 256 + tid;
-            As[buf][TA ? e % BK : e / BM][TA ? e / BK : e % BM] = ra[u];
+            const int kk = TA ? e % BK : e / BM;
+            T v = ra[u];
+            if (innd && k0 + kk < g.K) v = T((double)v * sk[k0 + kk]);
+            As[buf][kk][TA ? e / BK : e % BM] = v;
         }
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
@@ -291,19 +291,19 @@ __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs<T> g)
     const int nk = (g.K + BK - 1) / BK;
     fetch(0, ra0, rb0);
     if (nk > 1) fetch(BK, ra1, rb1);
-    stash(0, ra0, rb0);
+    stash(0, 0, ra0, rb0);
     __syncthreads();
     // invariant at the top of step ks: LDS buffer ks&1 holds step ks, register stage (ks+1)&1 holds
     // step ks+1 (if any), register stage ks&1 is free
     for (int ks = 0; ks < nk; ks += 2) {
         if (ks + 2 < nk) fetch((ks + 2) * BK, ra0, rb0);
         compute(0);
-        if (ks + 1 < nk) stash(1, ra1, rb1);
+        if (ks + 1 < nk) stash(1, (ks + 1) * BK, ra1, rb1);
         __syncthreads();
         if (ks + 1 >= nk) break;
         if (ks + 3 < nk) fetch((ks + 3) * BK, ra1, rb1);
         compute(1);
-        if (ks + 2 < nk) stash(0, ra0, rb0);
+        if (ks + 2 < nk) stash(0, (ks + 2) * BK, ra0, rb0);
         __syncthreads();
     }
 
