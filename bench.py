@@ -9,6 +9,8 @@ Workload (one "step" = one directional updating sweep of every chain of the batc
   B independent chains batched on one GPU.
 Metric: aggregate sweeps/s = B * steps / wall seconds of the timed region (inputs resident).
 Also measured in the same run:
+  * the whole sweep against the FP64 peak with the canonical algorithmic flop count,
+  * the drop-in application's sweeps/s at 8x8, beta=8, L=80 with all six observables,
   * the plain FP64 GPU form (1 chain, rank-1 Sherman-Morrison updates),
   * max |G - G_FP64| over the run (FP64 recompute at every stabilization),
   * the mixed FP32 form's |dG| on a short run (reported, not the value),
@@ -53,8 +55,16 @@ def _cpu_chain(args):
     return n, time.time() - t0, mkl
 
 
+def usable_cores():
+    """CPUs this process may run on (its affinity mask), not capped."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
 def cpu_baseline(seconds):
-    cores = min(os.cpu_count() or 1, 16)
+    cores = usable_cores()
     ctx = mp.get_context("spawn")
     t0 = time.time()
     with ctx.Pool(cores) as pool:
@@ -68,18 +78,101 @@ def cpu_baseline(seconds):
                       (cores, "MKL" if res[0][2] else "in-repo C", sweeps, wall)}
 
 
+def profile_value(name, key):
+    """A `key: value` line of a committed rocprofv3 summary under profiles/ (None if absent)."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith(key + ":"):
+                return float(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic():
     """HBM bytes per launch of the Metropolis kernel on this workload, from the committed
     rocprofv3 --pmc summary (FETCH_SIZE x fetch scale + WRITE_SIZE x write scale, the scales
-    from a device copy of known size)."""
-    path = os.path.join(ROOT, "profiles", "metropolis_kernel_n256_b256_pmc.md")
-    try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                return {"traffic": int(line.split(":")[1].strip()), "traffic_source": "profiles/" + os.path.basename(path)}
-    except OSError:
-        pass
-    return {}
+    from a device copy of known size), and its MFMA busy fraction from the MFMA counter run."""
+    out = {}
+    t = profile_value("metropolis_kernel_n256_b256_pmc.md", "traffic_bytes_per_launch")
+    if t is not None:
+        out["traffic"] = int(t)
+        out["traffic_source"] = "profiles/metropolis_kernel_n256_b256_pmc.md"
+    u = profile_value("mfma_f64_sweep_n256_b256_pmc.md", "metropolis_mfma_busy_frac")
+    if u is not None:
+        out["mfma_busy_frac"] = u
+        out["mfma_source"] = "profiles/mfma_f64_sweep_n256_b256_pmc.md"
+    return out
+
+
+def mixed_table(gpu, paces, chains, sweeps):
+    """Markdown table of the mixed-precision error max|G_mixed - G_FP64| against the stabilization
+    pace on the benchmark lattice (FP32 wraps and flushes, FP64 stabilization)."""
+    rows = ["| pace | stabilizations per sweep | max over chains and sweeps of max|G_mixed - G_FP64| |",
+            "|---|---|---|"]
+    for pace in paces:
+        e = gpu.Engine(NL, NT, BETA, U, mu=MU, pace=pace, n_chains=chains, precision=gpu.MIXED, delay_k=32)
+        for c in range(chains):
+            e.set_seed(c, BASE_SEED + c)
+            e.fields_random(c)
+        e.init()
+        run_sweeps(e, sweeps, True)
+        dG = max(e.stats(c)["max_abs_dG"] for c in range(chains))
+        e.close()
+        rows.append("| %d | %d | %.3e |" % (pace, -(-NT // pace), dG))
+        print("[bench] mixed pace %d: max|dG| %.3e" % (pace, dG), file=sys.stderr, flush=True)
+    return "\n".join(rows)
+
+
+def app_throughput():
+    """The drop-in application at the 8x8, U=-4, mu=-0.5, beta=8, L=80 (pace 10) configuration with
+    all six observables measured on the device: directional sweeps per second of its simulation
+    (warm-up, dynamic and down sweeps, measurements and binning), from the time it prints."""
+    import re
+    import subprocess
+    import tempfile
+    app = os.path.join(ROOT, "hubbard-attractive-u_amd", "hubbard-attractive-u")
+    if not os.path.exists(app):
+        return None
+    warm, bins, cap = 20, 2, 10
+    with tempfile.TemporaryDirectory() as d:
+        cfg = os.path.join(d, "c2.toml")
+        with open(cfg, "w") as f:
+            f.write("""[Model.HubbardAttractiveU]
+    nnt = 1.0
+    u = 4.0
+    mu = -0.5
+[Lattice.SquareLattice]
+    nl = 8
+    momentum_list = "AllKstars"
+[MonteCarlo]
+    beta = 8.0
+    nt = 80
+    stabilization_pace = 10
+[Measurement]
+    sweeps_warmup = %d
+    bin_num = %d
+    bin_capacity = %d
+    sweeps_between_bins = 0
+    observables = [ "FillingNumber", "DoubleOccupation", "StaticSWavePairingCorrelation",
+                    "DynamicGreenFunctions", "LocalDensityOfStates", "LocalDynamicTransverseSpinCorrelation" ]
+[Engine]
+    chains = 1
+    seed = %d
+    delay = 16
+""" % (warm, bins, cap, BASE_SEED))
+        r = subprocess.run([app, "-c", cfg, "-o", os.path.join(d, "out")], capture_output=True, text=True,
+                           timeout=600)
+        if r.returncode != 0:
+            return {"error": r.stderr[-300:]}
+        m = re.search(r"finished in (?:(\d+) m )?([0-9.]+) s", r.stdout)
+        if not m:
+            return None
+        secs = (int(m.group(1)) * 60 if m.group(1) else 0) + float(m.group(2))
+    sweeps = warm + bins * cap * 2   # updating warm-up, then per sample one dynamic and one down sweep
+    return {"sweeps_per_s": round(sweeps / secs, 2), "sweeps": sweeps, "seconds": secs,
+            "workload": "app: 8x8 U=-4 mu=-0.5 beta=8 L=80 pace 10, 1 chain, all six observables on the device"}
 
 
 # ----------------------------------------------------------------------------- GPU legs
@@ -123,6 +216,9 @@ def main():
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 (1 chain, k=1) leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision check leg")
+    ap.add_argument("--no-app", action="store_true", help="skip the drop-in application leg")
+    ap.add_argument("--mixed-table", action="store_true",
+                    help="only print the table of mixed-precision |dG| against the stabilization pace")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of 256 MiB (the reference kernel for rocprofv3 --pmc scales)")
     args = ap.parse_args()
@@ -130,6 +226,11 @@ def main():
         print("bench.py: this build runs on exactly one GPU; --gpus %d is not supported "
               "(chains are batched inside one device instead)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.mixed_table:
+        import dqmc_amd as gpu
+        print(mixed_table(gpu, (1, 2, 4, 8), 8, 2))
+        return
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
 
@@ -158,10 +259,10 @@ def main():
     # kernel, the path's largest single kernel, over two extra sweeps; the other timer classes
     # give the kernel-time shares
     qr_name = ("qr_v1_kernel" if os.environ.get("DQMC_QR", "") == "v1"
-               else "blocked QR (qr_panel4_kernel + qr_update_kernel per 32 columns)")
+               else "blocked QR and LU (panel + update kernels per 32 columns)")
     trsm_name = ("trsm_v1_kernel" if os.environ.get("DQMC_TRSM", "") == "v1"
                  else "blocked trsm (trsm_diag_kernel + gemm_kernel<double>)")
-    kernel_names = {0: "metropolis_kernel<double, 1024>", 1: qr_name, 2: "gemm_kernel<double> (stack/recompose)",
+    kernel_names = {0: "metropolis_kernel<double, 1024>", 1: qr_name, 2: "GEMMs (stack/recompose)",
                     3: trsm_name, 4: "kron_left/right_kernel<double>"}
     prof = {}
     for kid in (0, 1, 2, 3, 4):
@@ -174,18 +275,31 @@ def main():
     N = NL * NL
     ms, n, accs = prof[0]
     avg_s = ms / n / 1e3
-    # algorithmic work of one launch (one slice of every chain): the accepted flips' rank-1
-    # Sherman-Morrison updates, 2 N^2 flops each (src/hubbard.cpp:132-139)
+    # Algorithmic work of one launch (one slice of every chain): one read and one write of each
+    # chain's N x N FP64 G, and the accepted flips' rank-1 Sherman-Morrison updates, 2 N^2 flops
+    # each (src/hubbard.cpp:132-139).  The kernel moves several times the algorithmic bytes (the
+    # flushes of its site blocks), so HBM is what bounds it.
+    abytes = 2.0 * N * N * 8 * args.chains
     flops = 2.0 * N * N * accs / n
-    roof = {"bound": "mfma", "achieved": round(flops / avg_s / 1e12, 4), "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s"}
+    roof = {"bound": "hbm", "achieved": round(abytes / avg_s / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 5)
     roof["traffic"] = None
     roof.update(measured_traffic())
     roof["kernel"] = kernel_names[0]
     roof["avg_launch_us"] = round(avg_s * 1e6, 2)
+    roof["algorithmic_bytes_per_launch"] = round(abytes)
     roof["flops_per_launch"] = round(flops)
+    roof["achieved_tflops"] = round(flops / avg_s / 1e12, 3)
+    if roof.get("traffic"):
+        roof["traffic_gbs"] = round(roof["traffic"] / avg_s / 1e9, 1)
     roof["kernel_time_ms_2_sweeps"] = {kernel_names[k]: round(prof[k][0], 2) for k in prof}
+    # Whole sweep against the FP64 peak with the canonical algorithmic flop count per chain and
+    # sweep, F = (6 + 2a) L N^3 + (44/3) S N^3 (a = acceptance, S = stabilizations per sweep).
+    S = -(-NT // PACE)
+    fsweep = ((6.0 + 2.0 * acc) * NT + 44.0 / 3.0 * S) * float(N) ** 3 * args.chains
+    sweep_roof = {"flops_per_step": round(fsweep), "achieved_tflops": round(fsweep / (dt / args.steps) / 1e12, 3),
+                  "peak": PEAK_FP64_TFLOPS}
+    sweep_roof["frac"] = round(sweep_roof["achieved_tflops"] / PEAK_FP64_TFLOPS, 4)
     e.close()
 
     # denominator: plain FP64 GPU form, 1 chain, rank-1 updates
@@ -205,6 +319,11 @@ def main():
         mixed_rate, _ = timed(m, 8, 2, 0, False)
         mixed_dG = max(m.stats(c)["max_abs_dG"] for c in range(8))
         m.close()
+
+    app = None
+    if not args.no_app:
+        log("[bench] drop-in app, 8x8 beta=8 L=80, all observables")
+        app = app_throughput()
 
     out = {
         "metric": "aggregate sweeps/s at N=256, L=80: mixed+batched vs FP64 GPU, |dG|<=1e-6",
@@ -230,6 +349,8 @@ def main():
         "mixed_fp32_max_abs_dG": mixed_dG,
         "mixed_fp32_sweeps_per_s_8chains": round(mixed_rate, 3) if mixed_rate else None,
         "roofline": roof,
+        "sweep_roofline": sweep_roof,
+        "app_c2": app,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
