@@ -15,6 +15,7 @@
 #include <float.h>
 
 #include <cmath>
+#include <utility>
 
 namespace dqmc {
 
@@ -737,15 +738,17 @@ __device__ __forceinline__ double accept_prob(double gii, double delta, double* 
 }
 
 #define MB_MAX 32   // largest site block (delay) of the submatrix update
+#define MB_LD 33    // row stride of XS / YS; column MB_MAX holds 0.0
 
 // LDS layout of one Metropolis workgroup
 template <typename T>
 struct MetroLds {
     T* Xl;         // [kb][N]  X^T of the pending rank-a update (rows m, G-row index contiguous)
-    T* Yl;         // [kb][N]  Y (rows m, G-column index contiguous)
+    T* Yl;         // [kb][ys] Y (rows m; column j of row m at j ^ yswz(m))
+    int ys;        // row stride of Yl: N rounded up to 32
     double* colq;  // [32] column q of the current G(S,S), broadcast inside the site-loop wave
     double* rowq;  // [32] row q of the current G(S,S)
-    double* XS;    // [kb][kb]  gamma_m * G(S, i_m) of accepted flip m (restricted to the block)
+    double* XS;    // [MB_MAX][MB_LD]  gamma_m * G(S, i_m) of accepted flip m (restricted to the block)
     double* YS;    // [kb][kb]  G(i_m, S) - e_{i_m} (restricted to the block)
     double* unif;  // [N]
     double* gam;   // [kb]
@@ -760,13 +763,14 @@ __device__ MetroLds<T> metro_lds(unsigned char* smem, int N, int kb)
 {
     MetroLds<T> L;
     const int kb4 = ((kb + 3) / 4) * 4;
+    L.ys = ((N + 31) / 32) * 32;
     L.Xl = reinterpret_cast<T*>(smem);
     L.Yl = L.Xl + (size_t)kb4 * N;
-    L.colq = reinterpret_cast<double*>(L.Yl + (size_t)kb4 * N);
+    L.colq = reinterpret_cast<double*>(L.Yl + (size_t)kb4 * L.ys);
     L.rowq = L.colq + 32;
     L.XS = L.rowq + 32;
-    L.YS = L.XS + (size_t)kb * kb;
-    L.unif = L.YS + (size_t)kb * kb;
+    L.YS = L.XS + MB_MAX * MB_LD;
+    L.unif = L.YS + MB_MAX * MB_LD;
     L.gam = L.unif + N;
     L.sl = reinterpret_cast<int*>(L.gam + kb);
     L.ql = L.sl + N;
@@ -777,13 +781,20 @@ __device__ MetroLds<T> metro_lds(unsigned char* smem, int N, int kb)
 
 size_t metropolis_lds_bytes(int N, int kb, size_t esz)
 {
-    const int kb4 = ((kb + 3) / 4) * 4;
-    size_t b = (size_t)2 * kb4 * N * esz;
+    const int kb4 = ((kb + 3) / 4) * 4, ys = ((N + 31) / 32) * 32;
+    size_t b = (size_t)kb4 * (N + ys) * esz;
     b = (b + 7) / 8 * 8;
-    b += (size_t)8 * (64 + 2 * (size_t)kb * kb + N + kb);
+    b += (size_t)8 * (64 + 2 * MB_MAX * MB_LD + N + kb);
     b += (size_t)4 * (N + kb + 2) + (size_t)4 * ((N + 31) / 32) + 64;
     return b;
 }
+
+// Column slot of Y row c: element (c, j) of Yl sits at c * ys + (j ^ yswz(c)).  The map stays inside
+// the aligned 32-column block of j, so each column's slots belong to that column alone.  Two rows
+// c, c + 1 of the same pair land in opposite halves of the 64 LDS banks, so the flush's 16-column
+// reads of two rows at once, and the 16-lane x 2-row writes that stage rows, are free of bank
+// conflicts.
+__device__ __forceinline__ int yswz(int c) { return (c & ~1) ^ ((c & 1) << 4); }
 
 // Tile sets of the flush, in 16 x 16 tiles (tile column ct, tile row rt; nt per side).  The cross of
 // tile rows/columns tr0..tr1 holds the rows and columns of the next site block, which its site loop
@@ -826,7 +837,7 @@ __device__ __forceinline__ void flush_tile(int mode, int u, int nt, int tr0, int
 // latency overlaps; FT is 8, or 4 under the 128-register budget of a 1024-thread workgroup.
 template <typename T, int FT>
 __device__ __forceinline__ void flush_mfma(T* __restrict__ G, int N, const T* __restrict__ Xl, const T* __restrict__ Yl,
-                                           int a4, int mode, int tr0, int tr1, int w0)
+                                           int ys, int a4, int mode, int tr0, int tr1, int w0)
 {
     typedef typename Mfma<T>::acc_t acc_t;
     const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) - w0, nw = (blockDim.x >> 6) - w0;
@@ -858,7 +869,7 @@ __device__ __forceinline__ void flush_mfma(T* __restrict__ G, int N, const T* __
             for (int u = 0; u < FT; ++u) {
                 const int c0 = c0s[u], r0 = r0s[u];
                 const int ca = c0 + (lane & 15), gr = r0 + (lane & 15);
-                const T av = (ca < N) ? Yl[kk * N + ca] : T(0);   // A(row = G column, k)
+                const T av = (ca < N) ? Yl[kk * ys + (ca ^ yswz(kk))] : T(0);   // A(row = G column, k)
                 const T bv = (c0 < N && gr < N) ? Xl[kk * N + gr] : T(0);   // B(k, col = G row)
                 acc[u] = Mfma<T>::mma(av, bv, acc[u]);
             }
@@ -932,8 +943,8 @@ __device__ __forceinline__ void metro_site_block(const MetroLds<T>& L, const T* 
                 pr = accept_prob(dg, dr, &xr);
                 gr = dr / xr;
                 if (lane < bs) {
-                    L.XS[na * kb + lane] = xs;
-                    L.YS[na * kb + lane] = ysd;
+                    L.XS[na * MB_LD + lane] = xs;
+                    L.YS[na * MB_LD + lane] = ysd;
                 }
                 if (lane == 0) {
                     const int i = b0 + q;
@@ -950,6 +961,84 @@ __device__ __forceinline__ void metro_site_block(const MetroLds<T>& L, const T* 
         }
     }
     if (lane == 0) L.nacc[(b0 / kb) & 1] = na;
+}
+
+// acc + c[n] * x, where c[n] is lane n of the lane's 16-lane row (one v_fmac_f64_dpp with
+// row_newbcast; every lane of the wave must be active).  The s_nop covers the two wait states a
+// DPP read needs after a VALU write of its source, whatever the compiler placed before.
+template <int n>
+__device__ __forceinline__ double fmac_row_bcast(double acc, double c, double x);
+#define DQMC_FMAC_ROW_BCAST(n)                                                                            \
+    template <>                                                                                           \
+    __device__ __forceinline__ double fmac_row_bcast<n>(double acc, double c, double x)                   \
+    {                                                                                                     \
+        asm volatile("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:" #n " row_mask:0xf bank_mask:0xf" \
+                     : "+v"(acc)                                                                          \
+                     : "v"(c), "v"(x));                                                                   \
+        return acc;                                                                                       \
+    }
+DQMC_FMAC_ROW_BCAST(0)
+DQMC_FMAC_ROW_BCAST(1)
+DQMC_FMAC_ROW_BCAST(2)
+DQMC_FMAC_ROW_BCAST(3)
+DQMC_FMAC_ROW_BCAST(4)
+DQMC_FMAC_ROW_BCAST(5)
+DQMC_FMAC_ROW_BCAST(6)
+DQMC_FMAC_ROW_BCAST(7)
+DQMC_FMAC_ROW_BCAST(8)
+DQMC_FMAC_ROW_BCAST(9)
+DQMC_FMAC_ROW_BCAST(10)
+DQMC_FMAC_ROW_BCAST(11)
+DQMC_FMAC_ROW_BCAST(12)
+DQMC_FMAC_ROW_BCAST(13)
+DQMC_FMAC_ROW_BCAST(14)
+DQMC_FMAC_ROW_BCAST(15)
+#undef DQMC_FMAC_ROW_BCAST
+
+// Step M of the right-looking form of the rebuild recursions: every later term m' > M takes its
+// contribution from the finished term M, x[m'] = fma(c(m'), x[M], x[m']), with c(m') broadcast from
+// lane m' % 16 of ca (m' < 16) or cb (m' >= 16).  Term m' receives its contributions in the order
+// M = 0, 1, .., m'-1, the order of the left-looking sum x0[m'] + sum_{M<m'} x[M] c(m'), so both forms
+// give the same bits.
+template <int M, int P>
+__device__ __forceinline__ void rebuild_term(double (&x)[MB_MAX], double ca, double cb)
+{
+    if constexpr (P > M) x[P] = fmac_row_bcast<(P & 15)>(x[P], P < 16 ? ca : cb, x[M]);
+}
+template <int M, int... P>
+__device__ __forceinline__ void rebuild_step(double (&x)[MB_MAX], double ca, double cb, std::integer_sequence<int, P...>)
+{
+    (rebuild_term<M, P>(x, ca, cb), ...);
+}
+template <bool SCALE, int M>
+__device__ __forceinline__ void rebuild_one(double (&x)[MB_MAX], int na, const double* coef, int qa, int qb,
+                                            const double* gam, double& ca, double& cb, double& g)
+{
+    if (SCALE) x[M] = g * x[M];
+    double ta = 0.0, tb = 0.0, tg = 0.0;
+    if (M + 1 < na) {   // the next step's operands, in flight during this step's products
+        ta = coef[(M + 1) * MB_LD + qa];
+        tb = coef[(M + 1) * MB_LD + qb];
+        if (SCALE) tg = gam[M + 1];
+    }
+    rebuild_step<M>(x, ca, cb, std::make_integer_sequence<int, MB_MAX>{});
+    ca = ta;
+    cb = tb;
+    g = tg;
+}
+// The recursions of the delayed update, for one column per lane:
+//   X:  x[m] = gam[m] (x0[m] + sum_{m'<m} x[m'] YS[m'][q_m])      (coef = YS, SCALE)
+//   Y:  y[m] =         y0[m] + sum_{m'<m} XS[m'][q_m] y[m']       (coef = XS)
+// coef is the [MB_MAX][MB_LD] LDS table of the block, row = accepted flip, column = block position,
+// column MB_MAX = 0.0; qa / qb are q_{l % 16} and q_{16 + l % 16}, or MB_MAX past na, so that the
+// coefficients of terms m >= na are zero and those terms stay zero.
+template <bool SCALE, int... M>
+__device__ __forceinline__ void rebuild_chain(double (&x)[MB_MAX], int na, const double* coef, int qa, int qb,
+                                              const double* gam, std::integer_sequence<int, M...>)
+{
+    double ca = coef[qa], cb = coef[qb], g = SCALE ? gam[0] : 0.0;
+    bool go = true;
+    ((go = go && (M < na), go ? (rebuild_one<SCALE, M>(x, na, coef, qa, qb, gam, ca, cb, g), 0) : 0), ...);
 }
 
 // Submatrix Metropolis update of slice t.  Sites are visited in blocks S of kb consecutive
@@ -981,6 +1070,7 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
         L.unif[j] = canonical_from_draws(x1, x2);
     }
     for (int w = tid; w < nw; w += nth) L.mask[w] = 0u;
+    for (int w = tid; w < MB_MAX; w += nth) L.XS[w * MB_LD + MB_MAX] = L.YS[w * MB_LD + MB_MAX] = 0.0;
 
     const double dpos = a.delta[2 * b], dneg = a.delta[2 * b + 1];
     int nacc = 0;
@@ -1001,59 +1091,50 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
         const bool next = b1 < N;
         const int tr0 = next ? b1 / 16 : 0, tr1 = next ? (b1 + bs1 - 1) / 16 : 0;
         if (na > 0) {
-            // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  The two recursions
-            // are independent: with at least 2N threads one half builds X and the other half Y,
-            // each in registers with the recursion unrolled.  The Y threads read the block's rows
-            // G0(S, j) as one contiguous segment (staged in their own column j of Yl) instead of na
-            // scattered row reads.
-            const int a4 = ((na + 3) / 4) * 4;
-            const bool split = nth >= 2 * N;
-            for (int e = tid; e < (split ? 2 * N : N); e += nth) {
-                const int j = split ? (e % N) : e;
-                const bool do_x = !split || e < N, do_y = !split || e >= N;
-                if (do_x) {
-                    double xr[MB_MAX];
-#pragma unroll
-                    for (int m = 0; m < MB_MAX; ++m)
-                        xr[m] = (m < na) ? (double)G[j + (long long)(b0 + L.ql[m]) * N] : 0.0;
-#pragma unroll
-                    for (int m = 0; m < MB_MAX; ++m) {
-                        if (m < na) {
-                            const int qm = L.ql[m];
-                            double sx = xr[m];
-#pragma unroll
-                            for (int mp = 0; mp < m; ++mp) sx += xr[mp] * L.YS[mp * kb + qm];
-                            xr[m] = L.gam[m] * sx;
-                        }
-                    }
-#pragma unroll
-                    for (int m = 0; m < MB_MAX; ++m)
-                        if (m < a4) L.Xl[m * N + j] = T(xr[m]);
-                }
-                if (do_y) {
-                    for (int c = 0; c < bs; ++c) L.Yl[c * N + j] = G[(b0 + c) + (long long)j * N];
-                    double yr[MB_MAX];
-#pragma unroll
-                    for (int m = 0; m < MB_MAX; ++m)
-                        yr[m] = (m < na) ? (double)L.Yl[L.ql[m] * N + j] - (j == b0 + L.ql[m] ? 1.0 : 0.0) : 0.0;
+            // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  First the block's
+            // rows G0(S, :) are staged in Yl (16 lanes per column, two rows per lane: contiguous
+            // global reads).  Then each wave takes whole 64-column tasks, X or Y, and runs the
+            // recursion of its columns in registers with all 64 lanes active (the coefficients
+            // travel by DPP broadcast, which needs every source lane).
+            const int a4 = ((na + 3) / 4) * 4, ys = L.ys;
+            for (int w = tid; w < N * 16; w += nth) {
+                const int j = w >> 4, r = 2 * (w & 15);
+                if (r < bs) L.Yl[r * ys + (j ^ yswz(r))] = G[(b0 + r) + (long long)j * N];
+                if (r + 1 < bs) L.Yl[(r + 1) * ys + (j ^ yswz(r + 1))] = G[(b0 + r + 1) + (long long)j * N];
+            }
+            __syncthreads();
+            const int lane = tid & 63, nct = (N + 63) / 64, l16 = lane & 15;
+            const int qa = (l16 < na) ? L.ql[l16] : MB_MAX, qb = (16 + l16 < na) ? L.ql[16 + l16] : MB_MAX;
+            for (int task = tid >> 6; task < 2 * nct; task += nth >> 6) {
+                const bool isx = task < nct;
+                const int j = (isx ? task : task - nct) * 64 + lane;
+                const bool jin = j < N;
+                double xr[MB_MAX];
+                if (isx) {
 #pragma unroll
                     for (int m = 0; m < MB_MAX; ++m) {
-                        if (m < na) {
-                            const int qm = L.ql[m];
-                            double sy = yr[m];
-#pragma unroll
-                            for (int mp = 0; mp < m; ++mp) sy += L.XS[mp * kb + qm] * yr[mp];
-                            yr[m] = sy;
-                        }
+                        const int q = (m < na) ? L.ql[m] : 0;
+                        xr[m] = (m < na && jin) ? (double)G[j + (long long)(b0 + q) * N] : 0.0;
                     }
+                    rebuild_chain<true>(xr, na, L.YS, qa, qb, L.gam, std::make_integer_sequence<int, MB_MAX>{});
 #pragma unroll
                     for (int m = 0; m < MB_MAX; ++m)
-                        if (m < a4) L.Yl[m * N + j] = T(yr[m]);
+                        if (m < a4 && jin) L.Xl[m * N + j] = T(xr[m]);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < MB_MAX; ++m) {
+                        const int q = (m < na) ? L.ql[m] : 0;
+                        xr[m] = (m < na && jin) ? (double)L.Yl[q * ys + (j ^ yswz(q))] - (j == b0 + q ? 1.0 : 0.0) : 0.0;
+                    }
+                    rebuild_chain<false>(xr, na, L.XS, qa, qb, L.gam, std::make_integer_sequence<int, MB_MAX>{});
+#pragma unroll
+                    for (int m = 0; m < MB_MAX; ++m)
+                        if (m < a4 && jin) L.Yl[m * ys + (j ^ yswz(m))] = T(xr[m]);
                 }
             }
             __syncthreads();
             if (next) {
-                flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, a4, FLUSH_CROSS, tr0, tr1, 0);
+                flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, L.ys, a4, FLUSH_CROSS, tr0, tr1, 0);
                 __syncthreads();
             }
         }
@@ -1061,9 +1142,9 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
             if (tid < 64)
                 metro_site_block(L, G, f, N, kb, b1, bs1, dpos, dneg, health);
             else if (na > 0)
-                flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, ((na + 3) / 4) * 4, FLUSH_REST, tr0, tr1, 1);
+                flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, L.ys, ((na + 3) / 4) * 4, FLUSH_REST, tr0, tr1, 1);
         } else if (na > 0) {
-            flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, ((na + 3) / 4) * 4, FLUSH_ALL, 0, 0, 0);
+            flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, L.ys, ((na + 3) / 4) * 4, FLUSH_ALL, 0, 0, 0);
         }
         __syncthreads();
     }
