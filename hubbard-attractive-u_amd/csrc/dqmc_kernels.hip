@@ -743,7 +743,7 @@ __device__ __forceinline__ double accept_prob(double gii, double delta, double* 
 // LDS layout of one Metropolis workgroup
 template <typename T>
 struct MetroLds {
-    T* Xl;         // [kb][N]  X^T of the pending rank-a update (rows m, G-row index contiguous)
+    T* Xl;         // [kb][ys] X^T of the pending rank-a update (rows m; G row j of row m at j ^ yswz(m))
     T* Yl;         // [kb][ys] Y (rows m; column j of row m at j ^ yswz(m))
     int ys;        // row stride of Yl: N rounded up to 32
     double* colq;  // [32] column q of the current G(S,S), broadcast inside the site-loop wave
@@ -765,7 +765,7 @@ __device__ MetroLds<T> metro_lds(unsigned char* smem, int N, int kb)
     const int kb4 = ((kb + 3) / 4) * 4;
     L.ys = ((N + 31) / 32) * 32;
     L.Xl = reinterpret_cast<T*>(smem);
-    L.Yl = L.Xl + (size_t)kb4 * N;
+    L.Yl = L.Xl + (size_t)kb4 * L.ys;
     L.colq = reinterpret_cast<double*>(L.Yl + (size_t)kb4 * L.ys);
     L.rowq = L.colq + 32;
     L.XS = L.rowq + 32;
@@ -782,106 +782,104 @@ __device__ MetroLds<T> metro_lds(unsigned char* smem, int N, int kb)
 size_t metropolis_lds_bytes(int N, int kb, size_t esz)
 {
     const int kb4 = ((kb + 3) / 4) * 4, ys = ((N + 31) / 32) * 32;
-    size_t b = (size_t)kb4 * (N + ys) * esz;
+    size_t b = (size_t)kb4 * 2 * ys * esz;
     b = (b + 7) / 8 * 8;
     b += (size_t)8 * (64 + 2 * MB_MAX * MB_LD + N + kb);
     b += (size_t)4 * (N + kb + 2) + (size_t)4 * ((N + 31) / 32) + 64;
     return b;
 }
 
-// Column slot of Y row c: element (c, j) of Yl sits at c * ys + (j ^ yswz(c)).  The map stays inside
+// Slot of row c of Xl and Yl: element (c, j) sits at c * ys + (j ^ yswz(c)).  The map stays inside
 // the aligned 32-column block of j, so each column's slots belong to that column alone.  Two rows
-// c, c + 1 of the same pair land in opposite halves of the 64 LDS banks, so the flush's 16-column
+// c, c + 1 of the same pair land in opposite halves of the 64 LDS banks, so the flush's 16-element
 // reads of two rows at once, and the 16-lane x 2-row writes that stage rows, are free of bank
 // conflicts.
 __device__ __forceinline__ int yswz(int c) { return (c & ~1) ^ ((c & 1) << 4); }
 
-// Tile sets of the flush, in 16 x 16 tiles (tile column ct, tile row rt; nt per side).  The cross of
-// tile rows/columns tr0..tr1 holds the rows and columns of the next site block, which its site loop
-// and rebuild read; the rest is everything else.
-enum FlushTiles : int { FLUSH_ALL = 0, FLUSH_CROSS = 1, FLUSH_REST = 2 };
+// Tile sets of the flush, in 16 x 16 tiles (tile column ct, tile row rt; nt per side).  Tile rows and
+// columns tr0..tr1 hold the next site block; its diagonal block (the G(S, S) its site loop reads) is
+// FLUSH_DIAG, everything else FLUSH_OFFDIAG.  Consecutive tiles of a set share their tile column.
+enum FlushTiles : int { FLUSH_ALL = 0, FLUSH_DIAG = 1, FLUSH_OFFDIAG = 2 };
 __device__ __forceinline__ int flush_tile_count(int mode, int nt, int ncr)
 {
     if (mode == FLUSH_ALL) return nt * nt;
-    if (mode == FLUSH_CROSS) return ncr * nt + ncr * (nt - ncr);
-    return (nt - ncr) * (nt - ncr);
+    if (mode == FLUSH_DIAG) return ncr * ncr;
+    return nt * nt - ncr * ncr;
 }
 __device__ __forceinline__ void flush_tile(int mode, int u, int nt, int tr0, int ncr, int& ct, int& rt)
 {
     if (mode == FLUSH_ALL) {
         ct = u / nt;
         rt = u % nt;
-    } else if (mode == FLUSH_CROSS) {
-        if (u < ncr * nt) {
-            ct = tr0 + u / nt;
-            rt = u % nt;
-        } else {
-            const int v = u - ncr * nt, o = nt - ncr;
-            rt = tr0 + v / o;
-            ct = v % o;
-            if (ct >= tr0) ct += ncr;
-        }
-    } else {
-        const int o = nt - ncr;
-        ct = u / o;
-        rt = u % o;
+    } else if (mode == FLUSH_DIAG) {
+        ct = tr0 + u / ncr;
+        rt = tr0 + u % ncr;
+    } else if (u < (nt - ncr) * nt) {   // tile columns outside the block: every tile row
+        ct = u / nt;
+        rt = u % nt;
         if (ct >= tr0) ct += ncr;
+    } else {                            // tile columns of the block: tile rows outside it
+        const int v = u - (nt - ncr) * nt, o = nt - ncr;
+        ct = tr0 + v / o;
+        rt = v % o;
         if (rt >= tr0) rt += ncr;
     }
 }
 
-// G(ri, cj) += sum_m X[ri][m] Y[m][cj] over one tile set of the N x N matrix, on MFMA, by waves
-// w0 .. of the workgroup.  The tile is computed transposed (MFMA rows = G columns) so that each
+// G(ri, cj) += sum_m X[ri][m] Y[m][cj] over one tile set of the N x N matrix, on MFMA, by the nw
+// waves w0 .. w0 + nw - 1 of the workgroup.  The tile is computed transposed (MFMA rows = G columns) so that each
 // lane group touches contiguous column segments of the column-major G.  Each wave keeps FT tiles
 // in flight (all loads of a group are issued before its first MFMA) so that the global-memory
 // latency overlaps; FT is 8, or 4 under the 128-register budget of a 1024-thread workgroup.
 template <typename T, int FT>
 __device__ __forceinline__ void flush_mfma(T* __restrict__ G, int N, const T* __restrict__ Xl, const T* __restrict__ Yl,
-                                           int ys, int a4, int mode, int tr0, int tr1, int w0)
+                                           int ys, int a4, int mode, int tr0, int tr1, int w0, int nw)
 {
     typedef typename Mfma<T>::acc_t acc_t;
-    const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) - w0, nw = (blockDim.x >> 6) - w0;
+    const int lane = threadIdx.x & 63, wave = (threadIdx.x >> 6) - w0, l16 = lane & 15, lq = lane >> 4;
     const int nt = (N + 15) / 16, ncr = (mode == FLUSH_ALL) ? 0 : tr1 - tr0 + 1;
     const int ntt = flush_tile_count(mode, nt, ncr);
     for (int t0 = wave * FT; t0 < ntt; t0 += nw * FT) {
         acc_t acc[FT];
         int c0s[FT], r0s[FT];
+        bool ok[FT];
 #pragma unroll
         for (int u = 0; u < FT; ++u) {
             int ct = 0, rt = 0;
-            if (t0 + u < ntt) flush_tile(mode, t0 + u, nt, tr0, ncr, ct, rt);
-            c0s[u] = (t0 + u < ntt) ? ct * 16 : N;   // N marks an empty slot
-            r0s[u] = rt * 16;
+            ok[u] = t0 + u < ntt;
+            if (ok[u]) flush_tile(mode, t0 + u, nt, tr0, ncr, ct, rt);
+            c0s[u] = ok[u] ? ct * 16 : c0s[0];   // an empty slot repeats tile 0 and stores nothing
+            r0s[u] = ok[u] ? rt * 16 : r0s[0];
         }
 #pragma unroll
         for (int u = 0; u < FT; ++u) {
-            const int c0 = c0s[u], r0 = r0s[u];
-            const int gr = r0 + (lane & 15);
+            const int gr = r0s[u] + l16;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gc = c0 + Mfma<T>::row(lane, r);
+                const int gc = c0s[u] + Mfma<T>::row(lane, r);
                 acc[u][r] = (gr < N && gc < N) ? G[gr + (long long)gc * N] : T(0);
             }
         }
+        // Operands: A(row = G column ca, k) = Yl[k][ca], B(k, col = G row gr) = Xl[k][gr], both swizzled.
+        // Lanes past N read slots inside the padded rows; they only feed outputs that are not stored.
         for (int k0 = 0; k0 < a4; k0 += 4) {
-            const int kk = k0 + (lane >> 4);
+            const int kk = k0 + lq, sw = yswz(kk);
+            T av[FT], bv[FT];
 #pragma unroll
             for (int u = 0; u < FT; ++u) {
-                const int c0 = c0s[u], r0 = r0s[u];
-                const int ca = c0 + (lane & 15), gr = r0 + (lane & 15);
-                const T av = (ca < N) ? Yl[kk * ys + (ca ^ yswz(kk))] : T(0);   // A(row = G column, k)
-                const T bv = (c0 < N && gr < N) ? Xl[kk * N + gr] : T(0);   // B(k, col = G row)
-                acc[u] = Mfma<T>::mma(av, bv, acc[u]);
+                av[u] = Yl[kk * ys + ((c0s[u] + l16) ^ sw)];
+                bv[u] = Xl[kk * ys + ((r0s[u] + l16) ^ sw)];
             }
+#pragma unroll
+            for (int u = 0; u < FT; ++u) acc[u] = Mfma<T>::mma(av[u], bv[u], acc[u]);
         }
 #pragma unroll
         for (int u = 0; u < FT; ++u) {
-            const int c0 = c0s[u], r0 = r0s[u];
-            const int gr = r0 + (lane & 15);
+            const int gr = r0s[u] + l16;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gc = c0 + Mfma<T>::row(lane, r);
-                if (gr < N && gc < N) G[gr + (long long)gc * N] = acc[u][r];
+                const int gc = c0s[u] + Mfma<T>::row(lane, r);
+                if (ok[u] && gr < N && gc < N) G[gr + (long long)gc * N] = acc[u][r];
             }
         }
     }
@@ -995,9 +993,9 @@ DQMC_FMAC_ROW_BCAST(14)
 DQMC_FMAC_ROW_BCAST(15)
 #undef DQMC_FMAC_ROW_BCAST
 
-// Step M of the right-looking form of the rebuild recursions: every later term m' > M takes its
-// contribution from the finished term M, x[m'] = fma(c(m'), x[M], x[m']), with c(m') broadcast from
-// lane m' % 16 of ca (m' < 16) or cb (m' >= 16).  Term m' receives its contributions in the order
+// Step M of the right-looking form of the rebuild recursions: every later term m' > M (m' < na)
+// takes its contribution from the finished term M, x[m'] = fma(c(m'), x[M], x[m']), with c(m')
+// broadcast from lane m' % 16 of ca (m' < 16) or cb (m' >= 16).  Term m' receives its contributions in the order
 // M = 0, 1, .., m'-1, the order of the left-looking sum x0[m'] + sum_{M<m'} x[M] c(m'), so both forms
 // give the same bits.
 template <int M, int P>
@@ -1005,10 +1003,23 @@ __device__ __forceinline__ void rebuild_term(double (&x)[MB_MAX], double ca, dou
 {
     if constexpr (P > M) x[P] = fmac_row_bcast<(P & 15)>(x[P], P < 16 ? ca : cb, x[M]);
 }
-template <int M, int... P>
-__device__ __forceinline__ void rebuild_step(double (&x)[MB_MAX], double ca, double cb, std::integer_sequence<int, P...>)
+// Terms P0 .. P0+3 of step M; the whole group is skipped once P0 >= na (its terms stay zero).
+template <int M, int P0>
+__device__ __forceinline__ void rebuild_quad(double (&x)[MB_MAX], int na, double ca, double cb)
 {
-    (rebuild_term<M, P>(x, ca, cb), ...);
+    if constexpr (P0 + 3 > M) {
+        if (P0 < na) {
+            rebuild_term<M, P0>(x, ca, cb);
+            rebuild_term<M, P0 + 1>(x, ca, cb);
+            rebuild_term<M, P0 + 2>(x, ca, cb);
+            rebuild_term<M, P0 + 3>(x, ca, cb);
+        }
+    }
+}
+template <int M, int... Q>
+__device__ __forceinline__ void rebuild_step(double (&x)[MB_MAX], int na, double ca, double cb, std::integer_sequence<int, Q...>)
+{
+    (rebuild_quad<M, 4 * Q>(x, na, ca, cb), ...);
 }
 template <bool SCALE, int M>
 __device__ __forceinline__ void rebuild_one(double (&x)[MB_MAX], int na, const double* coef, int qa, int qb,
@@ -1021,7 +1032,7 @@ __device__ __forceinline__ void rebuild_one(double (&x)[MB_MAX], int na, const d
         tb = coef[(M + 1) * MB_LD + qb];
         if (SCALE) tg = gam[M + 1];
     }
-    rebuild_step<M>(x, ca, cb, std::make_integer_sequence<int, MB_MAX>{});
+    rebuild_step<M>(x, na, ca, cb, std::make_integer_sequence<int, MB_MAX / 4>{});
     ca = ta;
     cb = tb;
     g = tg;
@@ -1074,13 +1085,15 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
 
     const double dpos = a.delta[2 * b], dneg = a.delta[2 * b + 1];
     int nacc = 0;
-    // Software pipeline over the site blocks: after block B's rebuild, the flush first updates the
-    // cross of rows and columns of block B+1 (all waves); then wave 0 runs block B+1's site loop,
-    // which reads only that cross, while the other waves flush the rest of G.
+    // Software pipeline over the site blocks: after block B's rebuild, wave 0 applies B's flips to the
+    // diagonal block G(S', S') of block B+1 and runs B+1's site loop, which reads only that block,
+    // while the other waves apply them to the rest of G (the rows and columns of B+1 included, which
+    // B+1's rebuild reads after the next barrier).
     __syncthreads();
     uint32_t* health = a.health + b;
     if (tid < 64) metro_site_block(L, G, f, N, kb, 0, min(kb, N), dpos, dneg, health);
     __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6, nwv = nth >> 6, l16 = lane & 15;
     for (int b0 = 0; b0 < N; b0 += kb) {
         const int bs = min(kb, N - b0);
         // The count of block b0 sits in its own parity slot: when na == 0 no barrier separates this
@@ -1090,22 +1103,40 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
         const int b1 = b0 + kb, bs1 = (b1 < N) ? min(kb, N - b1) : 0;
         const bool next = b1 < N;
         const int tr0 = next ? b1 / 16 : 0, tr1 = next ? (b1 + bs1 - 1) / 16 : 0;
+        const int a4 = ((na + 3) / 4) * 4;
         if (na > 0) {
-            // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  First the block's
-            // rows G0(S, :) are staged in Yl (16 lanes per column, two rows per lane: contiguous
-            // global reads).  Then each wave takes whole 64-column tasks, X or Y, and runs the
-            // recursion of its columns in registers with all 64 lanes active (the coefficients
-            // travel by DPP broadcast, which needs every source lane).
-            const int a4 = ((na + 3) / 4) * 4, ys = L.ys;
-            for (int w = tid; w < N * 16; w += nth) {
-                const int j = w >> 4, r = 2 * (w & 15);
-                if (r < bs) L.Yl[r * ys + (j ^ yswz(r))] = G[(b0 + r) + (long long)j * N];
-                if (r + 1 < bs) L.Yl[(r + 1) * ys + (j ^ yswz(r + 1))] = G[(b0 + r + 1) + (long long)j * N];
+            // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  All waves first stage
+            // the block's rows G0(S, :) in Yl (16 lanes per column, two rows per lane: contiguous
+            // global reads, all in flight at once).  Then each wave takes whole 64-column tasks, X
+            // or Y, and runs the recursion of its columns in registers with all 64 lanes active (the
+            // coefficients travel by DPP broadcast, which needs every source lane).
+            const int ys = L.ys, nct = (N + 63) / 64;
+            const int qlane = (lane < na) ? L.ql[lane] : 0;   // lane m holds q_m
+            const int qa = (l16 < na) ? L.ql[l16] : MB_MAX, qb = (16 + l16 < na) ? L.ql[16 + l16] : MB_MAX;
+            {
+                constexpr int SI = (16 * 256 + MT - 1) / MT;   // staging items per thread at N <= 256
+                const int r = 2 * l16, r0 = min(r, bs - 1), r1 = min(r + 1, bs - 1);
+                T v0[SI], v1[SI];
+#pragma unroll
+                for (int i = 0; i < SI; ++i) {
+                    const int jj = min(((tid + i * nth) >> 4), N - 1);
+                    v0[i] = G[(b0 + r0) + (long long)jj * N];
+                    v1[i] = G[(b0 + r1) + (long long)jj * N];
+                }
+#pragma unroll
+                for (int i = 0; i < SI; ++i) {
+                    const int jj = (tid + i * nth) >> 4;
+                    if (jj < N && r < bs) L.Yl[r * ys + (jj ^ yswz(r))] = v0[i];
+                    if (jj < N && r + 1 < bs) L.Yl[(r + 1) * ys + (jj ^ yswz(r + 1))] = v1[i];
+                }
+                for (int w = tid + SI * nth; w < N * 16; w += nth) {   // workgroups narrower than MT
+                    const int jj = w >> 4;
+                    if (r < bs) L.Yl[r * ys + (jj ^ yswz(r))] = G[(b0 + r) + (long long)jj * N];
+                    if (r + 1 < bs) L.Yl[(r + 1) * ys + (jj ^ yswz(r + 1))] = G[(b0 + r + 1) + (long long)jj * N];
+                }
             }
             __syncthreads();
-            const int lane = tid & 63, nct = (N + 63) / 64, l16 = lane & 15;
-            const int qa = (l16 < na) ? L.ql[l16] : MB_MAX, qb = (16 + l16 < na) ? L.ql[16 + l16] : MB_MAX;
-            for (int task = tid >> 6; task < 2 * nct; task += nth >> 6) {
+            for (int task = wave; task < 2 * nct; task += nwv) {
                 const bool isx = task < nct;
                 const int j = (isx ? task : task - nct) * 64 + lane;
                 const bool jin = j < N;
@@ -1113,17 +1144,17 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
                 if (isx) {
 #pragma unroll
                     for (int m = 0; m < MB_MAX; ++m) {
-                        const int q = (m < na) ? L.ql[m] : 0;
+                        const int q = __builtin_amdgcn_readlane(qlane, m);
                         xr[m] = (m < na && jin) ? (double)G[j + (long long)(b0 + q) * N] : 0.0;
                     }
                     rebuild_chain<true>(xr, na, L.YS, qa, qb, L.gam, std::make_integer_sequence<int, MB_MAX>{});
 #pragma unroll
                     for (int m = 0; m < MB_MAX; ++m)
-                        if (m < a4 && jin) L.Xl[m * N + j] = T(xr[m]);
+                        if (m < a4 && jin) L.Xl[m * ys + (j ^ yswz(m))] = T(xr[m]);
                 } else {
 #pragma unroll
                     for (int m = 0; m < MB_MAX; ++m) {
-                        const int q = (m < na) ? L.ql[m] : 0;
+                        const int q = __builtin_amdgcn_readlane(qlane, m);
                         xr[m] = (m < na && jin) ? (double)L.Yl[q * ys + (j ^ yswz(q))] - (j == b0 + q ? 1.0 : 0.0) : 0.0;
                     }
                     rebuild_chain<false>(xr, na, L.XS, qa, qb, L.gam, std::make_integer_sequence<int, MB_MAX>{});
@@ -1133,18 +1164,21 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
                 }
             }
             __syncthreads();
-            if (next) {
-                flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, L.ys, a4, FLUSH_CROSS, tr0, tr1, 0);
-                __syncthreads();
-            }
         }
         if (next) {
-            if (tid < 64)
+            if (wave == 0) {
+                if (na > 0) {
+                    flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, L.ys, a4, FLUSH_DIAG, tr0, tr1, 0, 1);
+                    // this wave's stores of G(S', S') complete before its site loop reads them
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+                }
                 metro_site_block(L, G, f, N, kb, b1, bs1, dpos, dneg, health);
-            else if (na > 0)
-                flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, L.ys, ((na + 3) / 4) * 4, FLUSH_REST, tr0, tr1, 1);
+            } else if (na > 0) {
+                flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, L.ys, a4, FLUSH_OFFDIAG, tr0, tr1, 1, nwv - 1);
+            }
         } else if (na > 0) {
-            flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, L.ys, ((na + 3) / 4) * 4, FLUSH_ALL, 0, 0, 0);
+            flush_mfma<T, (MT >= 1024 ? 4 : 8)>(G, N, L.Xl, L.Yl, L.ys, a4, FLUSH_ALL, 0, 0, 0, nwv);
         }
         __syncthreads();
     }
@@ -1179,7 +1213,7 @@ hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, int threads, hipS
     if (shm > 160 * 1024) return hipErrorInvalidValue;
     if (a.N < 256) threads = ((a.N + 63) / 64) * 64 * 2 <= 1024 ? ((a.N + 63) / 64) * 64 * 2 : 1024;
     if (threads <= 256)
-        hipLaunchKernelGGL((metropolis_kernel<T, 256>), dim3(batch), dim3(threads < 64 ? 64 : threads), shm, s, a);
+        hipLaunchKernelGGL((metropolis_kernel<T, 256>), dim3(batch), dim3(threads < 128 ? 128 : threads), shm, s, a);
     else if (threads <= 512)
         hipLaunchKernelGGL((metropolis_kernel<T, 512>), dim3(batch), dim3(threads), shm, s, a);
     else
