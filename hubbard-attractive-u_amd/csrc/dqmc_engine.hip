@@ -74,6 +74,20 @@ struct dqmc_ctx {
     int* lu_perm = nullptr;          // [B][N] row order of the LU panel rows
     int metro_threads = 1024;  // Metropolis workgroup size at N = 256 (DQMC_METRO_THREADS)
     double *qrV = nullptr, *qrT = nullptr, *qrWt = nullptr;   // blocked-QR scratch: V, T, Wt and Wt T
+    // look-ahead of the blocked QR / LU (DQMC_LOOKAHEAD=0 turns it off): second stream, fork/join
+    // events and the parity-1 panel buffers
+    bool lookahead = true;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t la_fork = nullptr, la_join = nullptr;
+    double *qrV2 = nullptr, *qrT2 = nullptr;
+    int* lu_perm2 = nullptr;
+    LookAhead la() const
+    {
+        LookAhead a;
+        if (!lookahead) return a;
+        a.s2 = stream2; a.fork = la_fork; a.join = la_join; a.V2 = qrV2; a.T2 = qrT2; a.P2 = lu_perm2;
+        return a;
+    }
     int current_t = 0;
     long long updating_sweeps = 0;
     std::vector<double> chain_mu;
@@ -264,8 +278,9 @@ int gemm(dqmc_ctx* c, const T* A, long long sA, int lda, int tA, const T* B, lon
 hipError_t qr_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int batch, hipStream_t s)
 {
     if (c->qr_mode == 1) return launch_qr_v1(buf, stride, N, W, batch, s);
+    const LookAhead la = c->la();
     return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrT, c->qrWt, (long long)c->wdyn() * 32, c->tile64, c->qr_update,
-                             c->qr_panel, s);
+                             c->qr_panel, s, &la);
 }
 hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int c0, int P, int batch, hipStream_t s)
 {
@@ -279,7 +294,10 @@ hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, in
 hipError_t factor_any(dqmc_ctx* c, int W)
 {
     if (c->solve_lu)
-        return launch_lu_blocked(c->qr, c->sqr(), c->N, W, c->B, c->qrV, c->qrT, c->lu_perm, c->health, c->stream);
+    {
+        const LookAhead la = c->la();
+        return launch_lu_blocked(c->qr, c->sqr(), c->N, W, c->B, c->qrV, c->qrT, c->lu_perm, c->health, c->stream, &la);
+    }
     return qr_any(c, c->qr, c->sqr(), c->N, W, c->B, c->stream);
 }
 
@@ -832,6 +850,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         if (qp && std::string(qp) == "1024") c->qr_panel = QR_PANEL_1024;
         const char* qu = std::getenv("DQMC_QR_UPDATE");
         if (qu && std::string(qu) == "gemm") c->qr_update = QR_UPDATE_GEMM;
+        const char* lk = std::getenv("DQMC_LOOKAHEAD");
+        if (lk && std::string(lk) == "0") c->lookahead = false;
         const char* mt = std::getenv("DQMC_METRO_THREADS");
         if (mt) {
             const int v = std::atoi(mt);
@@ -854,6 +874,9 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     const int B = n_chains, L = c->L, S = c->S;
     const long long nn = c->NN();
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return c->fail(DQMC_EHIP, "stream");
+    if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) return c->fail(DQMC_EHIP, "stream");
+    HCHECK(c, hipEventCreateWithFlags(&c->la_fork, hipEventDisableTiming));
+    HCHECK(c, hipEventCreateWithFlags(&c->la_join, hipEventDisableTiming));
     HCHECK(c, prepare_kernels());
     for (auto& e : c->ev) HCHECK(c, hipEventCreate(&e));
     RCHECK(c->alloc(&c->expK, nn * 8)); RCHECK(c->alloc(&c->invExpK, nn * 8));
@@ -877,6 +900,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     RCHECK(c->alloc(&c->perm, (size_t)B * N * 4));
     RCHECK(c->alloc(&c->lu_perm, (size_t)B * N * 4));
     RCHECK(c->alloc(&c->qrV, (size_t)B * N * 32 * 8)); RCHECK(c->alloc(&c->qrT, (size_t)B * 32 * 32 * 8));
+    RCHECK(c->alloc(&c->qrV2, (size_t)B * N * 32 * 8)); RCHECK(c->alloc(&c->qrT2, (size_t)B * 32 * 32 * 8));
+    RCHECK(c->alloc(&c->lu_perm2, (size_t)B * N * 4));
     RCHECK(c->alloc(&c->qrWt, (size_t)2 * B * c->wdyn() * 32 * 8));
     for (DStack* s : {&c->left, &c->right}) {
         RCHECK(c->alloc(&s->U, (size_t)B * S * nn * 8));
@@ -943,6 +968,10 @@ void dqmc_destroy(dqmc_ctx* c)
     for (void* p : c->allocs) (void)hipFree(p);
     for (auto& e : c->ev) if (e) (void)hipEventDestroy(e);
     for (auto& e : c->kt_ev) (void)hipEventDestroy(e);
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
+    if (c->la_fork) (void)hipEventDestroy(c->la_fork);
+    if (c->la_join) (void)hipEventDestroy(c->la_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

@@ -119,15 +119,28 @@ hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, 
 // `panel`: QR_PANEL_256 (default) factors each panel with 4 waves per chain, a thread per row;
 // QR_PANEL_1024 with 16 waves, four threads per row.
 enum QrUpdate : int { QR_UPDATE_GEMM = 0, QR_UPDATE_REG = 2 };
+// Look-ahead of the blocked QR (QR_UPDATE_REG) and LU: with s2 set, the trailing update of panel k
+// first covers panel k + 1's 32 columns on the main stream; panel k + 1 then runs on the main
+// stream beside the rest of the update on s2 (fork and join through the two events, which a graph
+// capture of the main stream records as parallel branches).  V2 / T2 / P2 are the parity-1 panel
+// buffers (QR: V N*32 and T 32*32 doubles per chain; LU: L, L11^{-1} and the row order N ints).
+struct LookAhead {
+    hipStream_t s2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    double* V2 = nullptr;
+    double* T2 = nullptr;
+    int* P2 = nullptr;
+};
 enum QrPanel : int { QR_PANEL_256 = 0, QR_PANEL_1024 = 1 };
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
-                             double* Wt, long long wt_stride, int tile, int update, int panel, hipStream_t s);
+                             double* Wt, long long wt_stride, int tile, int update, int panel, hipStream_t s,
+                             const LookAhead* la = nullptr);
 // Blocked LU with partial pivoting of the leading N x N block of the same row-major N x W system
 // (N <= 256), applied to the W - N trailing columns: on exit the leading block holds U and the
 // trailing columns L^{-1} P rhs.  Scratch per chain: Ls N*32 doubles, Li 32*32 doubles, Ps N ints.
 // A zero pivot sets HEALTH_ZERO_PIVOT.
 hipError_t launch_lu_blocked(double* buf, long long stride, int N, int W, int batch, double* Ls, double* Li, int* Ps,
-                             uint32_t* health, hipStream_t s);
+                             uint32_t* health, hipStream_t s, const LookAhead* la = nullptr);
 // Blocked in-place back-substitution of the same system: 32-row diagonal solves (thread per
 // right-hand side) and MFMA GEMM updates of the rows above (`tile` = GEMM tile).
 hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,

@@ -1742,7 +1742,7 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs
 //      B-fragments of W2^T from LDS), and stores them.
 // At is read and written exactly once; V and T come from L2.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void qr_update_kernel(double* __restrict__ bufs, long long stride, int N,
-                                                            int W, int batch, int p0, int nb,
+                                                            int W, int batch, int p0, int nb, int cb0, int ncb,
                                                             const double* __restrict__ Vs,
                                                             const double* __restrict__ Vrows, long long vr_stride,
                                                             const double* __restrict__ Ts)
@@ -1753,7 +1753,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const int lr = lane >> 4, lc = lane & 15;
     const int R = N - p0, cols = W - (p0 + nb), nrt = (R + 15) >> 4;
     int b, cb;
-    chain_tile((cols + QRB - 1) / QRB, batch, b, cb);
+    chain_tile(ncb, batch, b, cb);
+    cb += cb0;
     const int c0 = cb * QRB;
     double* A = bufs + (long long)b * stride;
     const double* V = Vs + (long long)b * N * QRB;
@@ -1856,13 +1857,62 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 }
 
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
-                             double* Wt, long long wt_stride, int tile, int update, int panel, hipStream_t s)
+                             double* Wt, long long wt_stride, int tile, int update, int panel, hipStream_t s,
+                             const LookAhead* la)
 {
     // per chain: Vs N*QRB, Ts QRB*QRB, Wt 2 * wt_stride (Wt and Wt T; in QR_UPDATE_REG the first
-    // holds the row-major copy of V), wt_stride >= W*QRB
+    // holds the row-major copy of V, the second that of the look-ahead's parity-1 V), wt_stride >= W*QRB
     if (N > 256 || N < 1 || W <= N) return hipErrorInvalidValue;
     if (wt_stride < (long long)W * QRB) return hipErrorInvalidValue;
     double* Wt2 = Wt + (long long)batch * wt_stride;
+    auto launch_panel = [&](int p0, int par, hipStream_t st) {
+        const int nb = (N - p0 < QRB) ? N - p0 : QRB;
+        double *V = par ? la->V2 : Vs, *T = par ? la->T2 : Ts, *Vr = par ? Wt2 : Wt;
+        if (panel == QR_PANEL_1024)
+            hipLaunchKernelGGL(qr_panel4_kernel, dim3(batch), dim3(1024), 0, st, buf, stride, N, W, p0, nb, V, T, Vr,
+                               wt_stride);
+        else
+            hipLaunchKernelGGL(qr_panel_kernel, dim3(batch), dim3(256), 0, st, buf, stride, N, W, p0, nb, V, T, Vr,
+                               wt_stride);
+        return hipGetLastError();
+    };
+    auto launch_update = [&](int p0, int par, int cb0, int ncb, hipStream_t st) {
+        const int nb = (N - p0 < QRB) ? N - p0 : QRB;
+        double *V = par ? la->V2 : Vs, *T = par ? la->T2 : Ts, *Vr = par ? Wt2 : Wt;
+        hipLaunchKernelGGL(qr_update_kernel, dim3(ncb * batch), dim3(256), 0, st, buf, stride, N, W, batch, p0, nb,
+                           cb0, ncb, V, Vr, wt_stride, T);
+        return hipGetLastError();
+    };
+    if (la && la->s2 && update == QR_UPDATE_REG) {
+        // Look-ahead: the update of panel k first covers panel k + 1's columns (block 0); then panel
+        // k + 1 (main stream) runs beside the rest of the update (second stream).  Panel buffers
+        // alternate by parity, so panel k + 1 never overwrites what the rest of update k reads.
+        const int np = (N + QRB - 1) / QRB;
+        hipError_t e = launch_panel(0, 0, s);
+        if (e != hipSuccess) return e;
+        for (int k = 0; k < np; ++k) {
+            const int p0 = k * QRB, nb = (N - p0 < QRB) ? N - p0 : QRB;
+            const int cols = W - (p0 + nb), ncb = (cols + QRB - 1) / QRB;
+            if (ncb <= 0) continue;
+            if (k + 1 == np) {
+                if ((e = launch_update(p0, k & 1, 0, ncb, s)) != hipSuccess) return e;
+                continue;
+            }
+            if ((e = launch_update(p0, k & 1, 0, 1, s)) != hipSuccess) return e;
+            if (ncb > 1) {
+                if ((e = hipEventRecord(la->fork, s)) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(la->s2, la->fork, 0)) != hipSuccess) return e;
+            }
+            // the panel is submitted first so that its workgroups take their slots before the update's
+            if ((e = launch_panel(p0 + QRB, (k + 1) & 1, s)) != hipSuccess) return e;
+            if (ncb > 1 && (e = launch_update(p0, k & 1, 1, ncb - 1, la->s2)) != hipSuccess) return e;
+            if (ncb > 1) {
+                if ((e = hipEventRecord(la->join, la->s2)) != hipSuccess) return e;
+                if ((e = hipStreamWaitEvent(s, la->join, 0)) != hipSuccess) return e;
+            }
+        }
+        return hipSuccess;
+    }
     for (int p0 = 0; p0 < N; p0 += QRB) {
         const int nb = (N - p0 < QRB) ? N - p0 : QRB;
         const int R = N - p0, cols = W - (p0 + nb);
@@ -1878,7 +1928,7 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
         if (cols <= 0) continue;
         if (reg) {
             hipLaunchKernelGGL(qr_update_kernel, dim3(((cols + QRB - 1) / QRB) * batch), dim3(256), 0, s, buf,
-                               stride, N, W, batch, p0, nb, Vs, Wt, wt_stride, Ts);
+                               stride, N, W, batch, p0, nb, 0, (cols + QRB - 1) / QRB, Vs, Wt, wt_stride, Ts);
             e = hipGetLastError();
             if (e != hipSuccess) return e;
             continue;
@@ -2081,8 +2131,9 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs
 // and 1 hold the QRB panel rows A12: through LDS they form U12 = L11^{-1} A12, which every wave then
 // reads as B-fragments for A22 -= L21 U12 on its own row tiles.  The tile is read and written once.
 __global__ __launch_bounds__(256) void lu_update_kernel(double* __restrict__ bufs, long long stride, int N, int W,
-                                                        int batch, int p0, int nb, const double* __restrict__ Ls,
-                                                        const double* __restrict__ Li, const int* __restrict__ Ps)
+                                                        int batch, int p0, int nb, int cb0, int ncb,
+                                                        const double* __restrict__ Ls, const double* __restrict__ Li,
+                                                        const int* __restrict__ Ps)
 {
     __shared__ double S[2][QRB][QRB + 1];   // [0]: A12, [1]: U12
     typedef Mfma<double>::acc_t acc_t;
@@ -2090,7 +2141,8 @@ __global__ __launch_bounds__(256) void lu_update_kernel(double* __restrict__ buf
     const int lr = lane >> 4, lc = lane & 15;
     const int R = N - p0, cols = W - (p0 + nb), nrt = (R + 15) >> 4;
     int b, cb;
-    chain_tile((cols + QRB - 1) / QRB, batch, b, cb);
+    chain_tile(ncb, batch, b, cb);
+    cb += cb0;
     const int c0 = cb * QRB;
     double* A = bufs + (long long)b * stride;
     const double* L = Ls + (long long)b * N * QRB;
@@ -2161,19 +2213,48 @@ __global__ __launch_bounds__(256) void lu_update_kernel(double* __restrict__ buf
 }
 
 hipError_t launch_lu_blocked(double* buf, long long stride, int N, int W, int batch, double* Ls, double* Li, int* Ps,
-                             uint32_t* health, hipStream_t s)
+                             uint32_t* health, hipStream_t s, const LookAhead* la)
 {
     if (N > 256 || N < 1 || W <= N) return hipErrorInvalidValue;
-    for (int p0 = 0; p0 < N; p0 += QRB) {
+    const bool ahead = la && la->s2;
+    auto launch_panel = [&](int p0, int par, hipStream_t st) {
         const int nb = (N - p0 < QRB) ? N - p0 : QRB;
-        const int cols = W - (p0 + nb);
-        hipLaunchKernelGGL(lu_panel_kernel, dim3(batch), dim3(256), 0, s, buf, stride, N, W, p0, nb, Ls, Li, Ps, health);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(lu_update_kernel, dim3(((cols + QRB - 1) / QRB) * batch), dim3(256), 0, s, buf, stride, N,
-                           W, batch, p0, nb, Ls, Li, Ps);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(lu_panel_kernel, dim3(batch), dim3(256), 0, st, buf, stride, N, W, p0, nb,
+                           par ? la->V2 : Ls, par ? la->T2 : Li, par ? la->P2 : Ps, health);
+        return hipGetLastError();
+    };
+    auto launch_update = [&](int p0, int par, int cb0, int ncb, hipStream_t st) {
+        const int nb = (N - p0 < QRB) ? N - p0 : QRB;
+        hipLaunchKernelGGL(lu_update_kernel, dim3(ncb * batch), dim3(256), 0, st, buf, stride, N, W, batch, p0, nb,
+                           cb0, ncb, par ? la->V2 : Ls, par ? la->T2 : Li, par ? la->P2 : Ps);
+        return hipGetLastError();
+    };
+    const int np = (N + QRB - 1) / QRB;
+    hipError_t e = launch_panel(0, 0, s);
+    if (e != hipSuccess) return e;
+    for (int k = 0; k < np; ++k) {
+        const int p0 = k * QRB, nb = (N - p0 < QRB) ? N - p0 : QRB;
+        const int cols = W - (p0 + nb), ncb = (cols + QRB - 1) / QRB;
+        const int par = ahead ? (k & 1) : 0, npar = ahead ? ((k + 1) & 1) : 0;
+        if (!ahead || k + 1 == np) {
+            // in order: the whole update of panel k, then panel k + 1
+            if (ncb > 0 && (e = launch_update(p0, par, 0, ncb, s)) != hipSuccess) return e;
+            if (k + 1 < np && (e = launch_panel(p0 + QRB, npar, s)) != hipSuccess) return e;
+            continue;
+        }
+        // look-ahead, as in launch_qr_blocked: panel k + 1's columns first, then panel k + 1 beside
+        // the rest of the update (panel buffers and row orders alternate by parity)
+        if ((e = launch_update(p0, par, 0, 1, s)) != hipSuccess) return e;
+        if (ncb > 1) {
+            if ((e = hipEventRecord(la->fork, s)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(la->s2, la->fork, 0)) != hipSuccess) return e;
+        }
+        if ((e = launch_panel(p0 + QRB, npar, s)) != hipSuccess) return e;
+        if (ncb > 1 && (e = launch_update(p0, par, 1, ncb - 1, la->s2)) != hipSuccess) return e;
+        if (ncb > 1) {
+            if ((e = hipEventRecord(la->join, la->s2)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(s, la->join, 0)) != hipSuccess) return e;
+        }
     }
     return hipSuccess;
 }
