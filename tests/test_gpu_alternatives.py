@@ -17,6 +17,8 @@ one exercised).
   DQMC_TRSM=v1           column-parallel back-substitution instead of the blocked one
   DQMC_QR=v1             one-workgroup unblocked Householder QR instead of the blocked one
   DQMC_METRO_THREADS     Metropolis workgroup of 256 or 512 threads instead of 1024
+  DQMC_LOOKAHEAD=0       blocked QR and LU panels and updates in order on one stream, instead of
+                         each next panel beside the rest of the previous update on a second stream
 """
 import sys
 
@@ -59,7 +61,7 @@ def default_run():
 
 @pytest.mark.parametrize("var,val", [("DQMC_GEMM_TILE", "1"), ("DQMC_GEMM_TILE", "3"), ("DQMC_SOLVE", "qr"), ("DQMC_QR_PANEL", "1024"), ("DQMC_QR_UPDATE", "gemm"), ("DQMC_WRAP", "dense"), ("DQMC_TRSM", "v1"),
                                      ("DQMC_QR", "v1"), ("DQMC_METRO_THREADS", "256"),
-                                     ("DQMC_METRO_THREADS", "512")])
+                                     ("DQMC_METRO_THREADS", "512"), ("DQMC_LOOKAHEAD", "0")])
 def test_switch_follows_default_chains(default_run, monkeypatch, var, val):
     say("case: %s=%s, 16x16 L=40, 8 chains, 2 sweeps" % (var, val))
     monkeypatch.setenv(var, val)
