@@ -825,9 +825,12 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         return free_error(DQMC_EINVAL, "dqmc_create: need nl >= 1, nt >= 1, stabilization_pace >= 1, u >= 0, beta >= 0");
     if (precision != DQMC_FP64 && precision != DQMC_MIXED) return free_error(DQMC_EINVAL, "dqmc_create: bad precision");
     const int N = p->nl * p->nl;
-    if (p->nl > 16) return free_error(DQMC_EINVAL, "dqmc_create: nl > 16 is not supported (N <= 256)");
+    if (p->nl > 32) return free_error(DQMC_EINVAL, "dqmc_create: nl > 32 is not supported (N <= 1024)");
     if (delay_k < 1 || delay_k > 32) return free_error(DQMC_EINVAL, "dqmc_create: delay_k must be 1..32");
     const size_t esz = precision == DQMC_MIXED ? 4 : 8;
+    // The pending rank-k update of a site block lives in LDS: on lattices above 16 x 16 the delay is
+    // capped at what fits (the accept/reject sequence does not depend on it).
+    while (delay_k > 1 && metropolis_lds_bytes(N, delay_k, esz) > 160 * 1024) --delay_k;
     if (metropolis_lds_bytes(N, delay_k, esz) > 160 * 1024)
         return free_error(DQMC_EINVAL, "dqmc_create: the Metropolis block does not fit in LDS");
 
@@ -866,6 +869,18 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         c->trsm_v1 = tr && std::string(tr) == "v1";
         const char* w = std::getenv("DQMC_WRAP");
         c->kron = !(w && std::string(w) == "dense");
+        if (N > 256) {
+            // Lattices above 16 x 16: dense slice products (the ring factor is 16 x 16), the
+            // Householder QR solve with the wide panel kernel and the three-GEMM update
+            c->kron = false;
+            c->solve_lu = false;
+            c->qr_update = QR_UPDATE_GEMM;
+            c->lookahead = false;
+            if (c->qr_mode == 1) {
+                delete c;
+                return free_error(DQMC_EINVAL, "dqmc_create: DQMC_QR=v1 supports N <= 256 only");
+            }
+        }
     }
     c->chain_mu.assign(n_chains, p->mu);
     c->rng_host.assign(n_chains, 1ULL);

@@ -1,0 +1,619 @@
+// dqmc_gemm.hip -- batched GEMM on MFMA (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32) with fused
+// diagonal scalings (slice wraps, B-chain, UDV and recompose products), and the Kronecker-factored
+// hopping exponential on MFMA.
+#include "dqmc_device.h"
+
+namespace dqmc {
+
+// ---------------------------------------------------------------------------------------------
+//  GEMM.  256 threads = 4 waves in a 2 x 2 grid; each wave owns a (BM/2) x (BN/2) block of
+//  16 x 16 MFMA tiles.  K advances in steps of BK through LDS (k-major tiles so that the
+//  fragment reads of consecutive lanes hit consecutive LDS words).
+// ---------------------------------------------------------------------------------------------
+template <typename T, int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_kernel(GemmArgs<T> g)
+{
+    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+    constexpr int PAD = (sizeof(T) == 8) ? 1 : 2;
+    __shared__ T As[BK][BM + PAD];
+    __shared__ T Bs[BK][BN + PAD];
+    typedef typename Mfma<T>::acc_t acc_t;
+
+    const int gm = (g.M + BM - 1) / BM, gn = (g.Nc + BN - 1) / BN;
+    int b, t;
+    chain_tile(gm * gn, g.batch, b, t);
+    const int m0 = (t % gm) * BM, n0 = (t / gm) * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const T* __restrict__ A = g.A + (long long)b * g.sA;
+    const T* __restrict__ B = g.B + (long long)b * g.sB;
+    T* __restrict__ C = g.C + (long long)b * g.sC;
+
+    acc_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    // The next K tile is fetched into registers while the current one is multiplied from LDS.
+    constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+    T ra[NA], rb[NB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = u * 256 + tid;
+            int i, kk;
+            if (!g.tA) { i = e % BM; kk = e / BM; } else { kk = e % BK; i = e / BK; }
+            const int gi = m0 + i, gk = k0 + kk;
+            T v = T(0);
+            if (gi < g.M && gk < g.K) {
+                v = g.tA ? A[gk + (long long)gi * g.lda] : A[gi + (long long)gk * g.lda];
+                if (g.innd.mode != DIAG_NONE) v = T((double)v * diag_value(g.innd, b, gk));
+            }
+            ra[u] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = u * 256 + tid;
+            int j, kk;
+            if (!g.tB) { kk = e % BK; j = e / BK; } else { j = e % BN; kk = e / BN; }
+            const int gj = n0 + j, gk = k0 + kk;
+            T v = T(0);
+            if (gj < g.Nc && gk < g.K) v = g.tB ? B[gj + (long long)gk * g.ldb] : B[gk + (long long)gj * g.ldb];
+            rb[u] = v;
+        }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = u * 256 + tid;
+            int i, kk;
+            if (!g.tA) { i = e % BM; kk = e / BM; } else { kk = e % BK; i = e / BK; }
+            As[kk][i] = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = u * 256 + tid;
+            int j, kk;
+            if (!g.tB) { kk = e % BK; j = e / BK; } else { j = e % BN; kk = e / BN; }
+            Bs[kk][j] = rb[u];
+        }
+        __syncthreads();
+        if (k0 + BK < g.K) fetch(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T af[TM], bf[TN];
+#pragma unroll
+            for (int i = 0; i < TM; ++i) af[i] = As[kk + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < TN; ++j) bf[j] = Bs[kk + (lane >> 4)][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm * WM + i * 16 + Mfma<T>::row(lane, r);
+                const int gj = n0 + wn * WN + j * 16 + (lane & 15);
+                if (gi < g.M && gj < g.Nc) {
+                    double v = (double)acc[i][j][r] * g.alpha;
+                    if (g.rowd.mode != DIAG_NONE) v *= diag_value(g.rowd, b, gi);
+                    if (g.cold.mode != DIAG_NONE) v *= diag_value(g.cold, b, gj);
+                    const long long off = g.tC ? (gj + (long long)gi * g.ldc) : (gi + (long long)gj * g.ldc);
+                    if (g.beta != 0.0) v += g.beta * (double)C[off];
+                    C[off] = T(v);
+                }
+            }
+}
+
+// 64 x 64 output tile with compile-time operand layouts, LDS tiles double-buffered (one barrier
+// per K step) and the global loads two K steps ahead in two register stages, so a load has the
+// MFMA work of two steps to arrive.  The loop runs two steps per iteration so that every register
+// stage is addressed statically.
+template <typename T, bool TA, bool TB>
+__global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs<T> g)
+{
+    constexpr int BM = 64, BN = 64, BK = 16, WM = 32, WN = 32, NA = BM * BK / 256, NB = BN * BK / 256;
+    constexpr int PAD = (sizeof(T) == 8) ? 1 : 2;
+    __shared__ T As[2][BK][BM + PAD];
+    __shared__ T Bs[2][BK][BN + PAD];
+    __shared__ double sk[256], srow[BM], scol[BN];   // diagonal scalings, evaluated once per tile
+    typedef typename Mfma<T>::acc_t acc_t;
+
+    const int gm = (g.M + BM - 1) / BM, gn = (g.Nc + BN - 1) / BN;
+    int b, t;
+    chain_tile(gm * gn, g.batch, b, t);
+    const int m0 = (t % gm) * BM, n0 = (t / gm) * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const T* __restrict__ A = g.A + (long long)b * g.sA;
+    const T* __restrict__ B = g.B + (long long)b * g.sB;
+    T* __restrict__ C = g.C + (long long)b * g.sC;
+    const bool innd = g.innd.mode != DIAG_NONE;
+    if (innd)
+        for (int kk = tid; kk < g.K && kk < 256; kk += 256) sk[kk] = diag_value(g.innd, b, kk);
+    if (tid < BM) srow[tid] = (g.rowd.mode != DIAG_NONE && m0 + tid < g.M) ? diag_value(g.rowd, b, m0 + tid) : 1.0;
+    else if (tid < BM + BN)
+        scol[tid - BM] = (g.cold.mode != DIAG_NONE && n0 + tid - BM < g.Nc) ? diag_value(g.cold, b, n0 + tid - BM) : 1.0;
+    __syncthreads();
+
+    acc_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    T ra0[NA], rb0[NB], ra1[NA], rb1[NB];
+    auto fetch = [&](int k0, T (&ra)[NA], T (&rb)[NB]) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = u * 256 + tid;
+            const int i = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
+            const int gi = m0 + i, gk = k0 + kk;
+            T v = T(0);
+            if (gi < g.M && gk < g.K) v = TA ? A[gk + (long long)gi * g.lda] : A[gi + (long long)gk * g.lda];
+            ra[u] = v;   // the inner scaling is applied at the LDS store, so the load is not waited on here
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = u * 256 + tid;
+            const int j = TB ? e % BN : e / BK, kk = TB ? e / BN : e % BK;
+            const int gj = n0 + j, gk = k0 + kk;
+            rb[u] = (gj < g.Nc && gk < g.K) ? (TB ? B[gj + (long long)gk * g.ldb] : B[gk + (long long)gj * g.ldb]) : T(0);
+        }
+    };
+    auto stash = [&](int buf, int k0, const T (&ra)[NA], const T (&rb)[NB]) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = u * 256 + tid;
+            const int kk = TA ? e % BK : e / BM;
+            T v = ra[u];
+            if (innd && k0 + kk < g.K) v = T((double)v * sk[k0 + kk]);
+            As[buf][kk][TA ? e / BK : e % BM] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = u * 256 + tid;
+            Bs[buf][TB ? e / BN : e % BK][TB ? e % BN : e / BK] = rb[u];
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            T af[2], bf[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) af[i] = As[buf][kk + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) bf[j] = Bs[buf][kk + (lane >> 4)][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = Mfma<T>::mma(af[i], bf[j], acc[i][j]);
+        }
+    };
+
+    const int nk = (g.K + BK - 1) / BK;
+    fetch(0, ra0, rb0);
+    if (nk > 1) fetch(BK, ra1, rb1);
+    stash(0, 0, ra0, rb0);
+    __syncthreads();
+    // invariant at the top of step ks: LDS buffer ks&1 holds step ks, register stage (ks+1)&1 holds
+    // step ks+1 (if any), register stage ks&1 is free
+    for (int ks = 0; ks < nk; ks += 2) {
+        if (ks + 2 < nk) fetch((ks + 2) * BK, ra0, rb0);
+        compute(0);
+        if (ks + 1 < nk) stash(1, (ks + 1) * BK, ra1, rb1);
+        __syncthreads();
+        if (ks + 1 >= nk) break;
+        if (ks + 3 < nk) fetch((ks + 3) * BK, ra1, rb1);
+        compute(1);
+        if (ks + 2 < nk) stash(0, (ks + 2) * BK, ra0, rb0);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm * WM + i * 16 + Mfma<T>::row(lane, r);
+                const int gj = n0 + wn * WN + j * 16 + (lane & 15);
+                if (gi < g.M && gj < g.Nc) {
+                    double v = (double)acc[i][j][r] * g.alpha;
+                    if (g.rowd.mode != DIAG_NONE) v *= srow[gi - m0];
+                    if (g.cold.mode != DIAG_NONE) v *= scol[gj - n0];
+                    const long long off = g.tC ? (gj + (long long)gi * g.ldc) : (gi + (long long)gj * g.ldc);
+                    if (g.beta != 0.0) v += g.beta * (double)C[off];
+                    C[off] = T(v);
+                }
+            }
+}
+
+template <typename T, bool TA, bool TB>
+static hipError_t launch_gemm64(const GemmArgs<T>& a, hipStream_t s)
+{
+    const long long blocks = (long long)((a.M + 63) / 64) * ((a.Nc + 63) / 64) * a.batch;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm64_kernel<T, TA, TB>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, int BM, int BN>
+static hipError_t launch_gemm_tile(const GemmArgs<T>& a, hipStream_t s)
+{
+    const long long blocks = (long long)((a.M + BM - 1) / BM) * ((a.Nc + BN - 1) / BN) * a.batch;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, 16>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_gemm(const GemmArgs<T>& a, int tile, hipStream_t s)
+{
+    if (a.batch <= 0 || a.M <= 0 || a.Nc <= 0) return hipSuccess;
+    switch (tile) {
+        case TILE_32x32: return launch_gemm_tile<T, 32, 32>(a, s);
+        case TILE_64x32: return launch_gemm_tile<T, 64, 32>(a, s);
+        case TILE_64x64_PF:
+            if (a.innd.mode != DIAG_NONE && a.K > 256) return launch_gemm_tile<T, 64, 64>(a, s);
+            if (a.tA) return a.tB ? launch_gemm64<T, true, true>(a, s) : launch_gemm64<T, true, false>(a, s);
+            return a.tB ? launch_gemm64<T, false, true>(a, s) : launch_gemm64<T, false, false>(a, s);
+        default: return launch_gemm_tile<T, 64, 64>(a, s);
+    }
+}
+template hipError_t launch_gemm<double>(const GemmArgs<double>&, int, hipStream_t);
+template hipError_t launch_gemm<float>(const GemmArgs<float>&, int, hipStream_t);
+
+// ---------------------------------------------------------------------------------------------
+//  Kronecker-factored hopping exponential on MFMA.
+//  A vector v of the lattice (site i = x + nl*y) is an nl x nl tile V(x, y).  (E(x)E) v is the
+//  tile E V E^T = E V E (E symmetric).  With 16x16x4 MFMA fragments:
+//    step 1  Y^T = V^T E^T : A-fragment = V^T read from memory, B-fragment = E (constant);
+//    step 2  Z   = Y E     : the A-fragment of step k is exactly item k of step 1's accumulator
+//  (the f64 accumulator layout puts Y^T(4k + l/16, l%16) in lane l, item k), so the tile never
+//  leaves registers.  8 MFMAs per lattice vector.
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ int kron_row(int lane, int r) { return Mfma<T>::row(lane, r); }
+
+// Transform one tile: vin(x', y') supplied by `ld(x', y')`, result passed to `st(x, y, value)`.
+template <typename T, typename LD, typename ST>
+__device__ __forceinline__ void kron_tile(const T (&ef)[4], const T (&ef2)[4], int lane, int nl, LD ld, ST st)
+{
+    typedef typename Mfma<T>::acc_t acc_t;
+    acc_t y = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int xp = (lane >> 4) + 4 * s, yp = lane & 15;   // A(row = y', k = x') = V(x', y')
+        const T v = (xp < nl && yp < nl) ? ld(xp, yp) : T(0);
+        y = Mfma<T>::mma(v, ef[s], y);                          // B(k = x', col = x) = E[x][x']
+    }
+    acc_t z = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) z = Mfma<T>::mma(y[s], ef2[s], z);   // A(row = x, k = y') = Y(x, y')
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int x = kron_row<T>(lane, r), yy = lane & 15;
+        if (x < nl && yy < nl) st(x, yy, z[r]);
+    }
+}
+
+// E fragments B(k, col) = E[col][k], col = l%16.  Step 1 contracts k = l/16 + 4s.  Step 2
+// contracts over the rows of step 1's accumulator, item s of lane l: k = Mfma<T>::row(l, s),
+// i.e. 4s + l/16 for f64 and 4(l/16) + s for f32.
+template <typename T>
+__device__ __forceinline__ void load_efrag(const T* E, int lane, T (&ef)[4], T (&ef2)[4])
+{
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        ef[s] = E[(lane & 15) * 16 + (lane >> 4) + 4 * s];
+        ef2[s] = E[(lane & 15) * 16 + Mfma<T>::row(lane, s)];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void kron_left_kernel(KronArgs<T> a)
+{
+    const int b = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nl = a.nl, N = a.N;
+    T ef[4], ef2[4];
+    load_efrag<T>(a.E, lane, ef, ef2);
+    const T* X = a.X + (long long)b * a.sX;
+    T* C = a.C + (long long)b * a.sC;
+    const int cbase = blockIdx.x * 16 + wave * 4;
+    // The diagonal scalings depend on the lattice site only, so each lane evaluates the ones of
+    // its four input sites and four output sites once for all columns.
+    double din[4], dout[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int xp = (lane >> 4) + 4 * s, yp = lane & 15;
+        din[s] = (a.innd.mode != DIAG_NONE && xp < nl && yp < nl) ? diag_value(a.innd, b, xp + nl * yp) : 1.0;
+        const int x = kron_row<T>(lane, s), y = lane & 15;
+        dout[s] = (a.rowd.mode != DIAG_NONE && x < nl && y < nl) ? diag_value(a.rowd, b, x + nl * y) : 1.0;
+    }
+    // all four lattice vectors of the wave are loaded before the first MFMA
+    T vin[4][4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        const int c = cbase + cc;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int xp = (lane >> 4) + 4 * s, yp = lane & 15;
+            T v = T(0);
+            if (c < N && xp < nl && yp < nl) {
+                v = X[(long long)c * N + xp + nl * yp];
+                if (a.innd.mode != DIAG_NONE) v = T((double)v * din[s]);
+            }
+            vin[cc][s] = v;
+        }
+    }
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        const int c = cbase + cc;
+        if (c >= N) break;
+        const double cs = (a.cold.mode != DIAG_NONE) ? diag_value(a.cold, b, c) : 1.0;
+        T* cc_out = C + (long long)c * N;
+        typedef typename Mfma<T>::acc_t acc_t;
+        acc_t y = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) y = Mfma<T>::mma(vin[cc][s], ef[s], y);
+        acc_t z = acc_t{0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) z = Mfma<T>::mma(y[s], ef2[s], z);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int x = kron_row<T>(lane, r), yy = lane & 15;
+            if (x < nl && yy < nl) cc_out[x + nl * yy] = T((double)z[r] * cs * dout[r]);
+        }
+    }
+}
+
+// Right multiplication: 16 consecutive rows per workgroup are staged through LDS (coalesced
+// column segments), transformed per row, and written back.
+template <typename T>
+__global__ __launch_bounds__(256) void kron_right_kernel(KronArgs<T> a)
+{
+    __shared__ T tile[256][17];
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nl = a.nl, N = a.N;
+    const int r0 = blockIdx.x * 16;
+    const T* X = a.X + (long long)b * a.sX;
+    T* C = a.C + (long long)b * a.sC;
+    for (int e = tid; e < N * 16; e += 256) {
+        const int rr = e & 15, c = e >> 4;
+        const int row = r0 + rr;
+        T v = T(0);
+        if (row < N) {
+            v = X[row + (long long)c * N];
+            if (a.innd.mode != DIAG_NONE) v = T((double)v * diag_value(a.innd, b, c));
+        }
+        tile[c][rr] = v;
+    }
+    __syncthreads();
+    T ef[4], ef2[4];
+    load_efrag<T>(a.E, lane, ef, ef2);
+    for (int rr = wave; rr < 16; rr += 4) {
+        T vin[4];
+        // read the whole row first: the tile is transformed in place
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const int xp = (lane >> 4) + 4 * s, yp = lane & 15;
+            vin[s] = (xp < nl && yp < nl) ? tile[xp + nl * yp][rr] : T(0);
+        }
+        kron_tile<T>(ef, ef2, lane, nl,
+            [&](int xp, int yp) {
+                return vin[(xp - (lane >> 4)) >> 2];
+            },
+            [&](int x, int y, T z) { tile[x + nl * y][rr] = z; });
+    }
+    __syncthreads();
+    for (int e = tid; e < N * 16; e += 256) {
+        const int rr = e & 15, c = e >> 4;
+        const int row = r0 + rr;
+        if (row < N) {
+            double v = (double)tile[c][rr];
+            if (a.rowd.mode != DIAG_NONE) v *= diag_value(a.rowd, b, row);
+            if (a.cold.mode != DIAG_NONE) v *= diag_value(a.cold, b, c);
+            C[row + (long long)c * N] = T(v);
+        }
+    }
+}
+
+// 16 x 16 lattice (N = 256): a lattice vector read as 64 lanes x 4 consecutive sites,
+// site = l + 64 s, is exactly the A-fragment V(x = l%16, y = 4 s + l/16) of step 1, so
+//   step 1  Y = V E^T   (A = V from a coalesced load, B(y, j) = E(j, y))
+//   step 2  Z = E Y     (A(i, x) = E(i, x), B = step 1's accumulator items)
+// and Z = E V E^T leaves the accumulator as Z(row(l, r), l%16), which a padded LDS transpose turns
+// back into consecutive sites for a coalesced store.
+template <typename T>
+__device__ __forceinline__ void kron16_vec(const T (&ef)[4], const T (&ef2)[4], const T (&v)[4],
+                                           typename Mfma<T>::acc_t& z)
+{
+    typedef typename Mfma<T>::acc_t acc_t;
+    acc_t y = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) y = Mfma<T>::mma(v[s], ef[s], y);
+    z = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) z = Mfma<T>::mma(ef2[s], y[s], z);
+}
+
+// Left product on the 16 x 16 lattice: each wave transforms 4 columns of X, one after the other
+// through its own transpose buffer.
+template <typename T>
+__global__ __launch_bounds__(256) void kron_left16_kernel(KronArgs<T> a)
+{
+    __shared__ T zt[4][16][17];   // per wave: Z(i, j) at [j][i]
+    const int b = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T ef[4], ef2[4];
+    load_efrag<T>(a.E, lane, ef, ef2);
+    const T* X = a.X + (long long)b * a.sX;
+    T* C = a.C + (long long)b * a.sC;
+    const int cbase = blockIdx.x * 16 + wave * 4;
+    double din[4], dout[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int site = lane + 64 * s;
+        din[s] = (a.innd.mode != DIAG_NONE) ? diag_value(a.innd, b, site) : 1.0;
+        dout[s] = (a.rowd.mode != DIAG_NONE) ? diag_value(a.rowd, b, site) : 1.0;
+    }
+    T vin[4][4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const T v = X[(long long)(cbase + cc) * 256 + lane + 64 * s];
+            vin[cc][s] = (a.innd.mode != DIAG_NONE) ? T((double)v * din[s]) : v;
+        }
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        typename Mfma<T>::acc_t z;
+        kron16_vec<T>(ef, ef2, vin[cc], z);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) zt[wave][lane & 15][Mfma<T>::row(lane, r)] = z[r];
+        wave_lds_sync();
+        const int c = cbase + cc;
+        const double cs = (a.cold.mode != DIAG_NONE) ? diag_value(a.cold, b, c) : 1.0;
+        T zv[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) zv[s] = zt[wave][(lane >> 4) + 4 * s][lane & 15];
+        wave_lds_sync();   // the buffer is read before the next column overwrites it
+#pragma unroll
+        for (int s = 0; s < 4; ++s)   // site l + 64 s = i + 16 j with i = l%16, j = l/16 + 4 s
+            C[(long long)c * 256 + lane + 64 * s] = T((double)zv[s] * cs * dout[s]);
+    }
+}
+
+// Right product on the 16 x 16 lattice: 32 rows of X per workgroup (every column read and written
+// as one 256-byte segment, two rows per 16-byte access), staged row-major in LDS with row stride
+// 273 and one pad word per 16 sites so that both the lattice-vector reads and the transposed
+// accumulator writes spread over the banks; wave w of 8 transforms rows w, w+8, w+16, w+24.
+#define KR16_LD 273
+#define KR16_ROWS 32
+__device__ __forceinline__ int kr16_pad(int site) { return site + (site >> 4); }
+template <typename T>
+__global__ __launch_bounds__(512) void kron_right16_kernel(KronArgs<T> a)
+{
+    __shared__ T tile[KR16_ROWS * KR16_LD];
+    const int b = blockIdx.y, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = blockIdx.x * KR16_ROWS;
+    const T* X = a.X + (long long)b * a.sX;
+    T* C = a.C + (long long)b * a.sC;
+#pragma unroll 4
+    for (int e = tid; e < KR16_ROWS / 2 * 256; e += 512) {
+        const int rr = 2 * (e & 15), c = e >> 4;
+        const T* src = X + r0 + rr + (long long)c * 256;
+        T v0, v1;
+        if (sizeof(T) == 8) {
+            const double2 v = *reinterpret_cast<const double2*>(src);
+            v0 = T(v.x); v1 = T(v.y);
+        } else {
+            const float2 v = *reinterpret_cast<const float2*>(src);
+            v0 = T(v.x); v1 = T(v.y);
+        }
+        if (a.innd.mode != DIAG_NONE) {
+            const double d = diag_value(a.innd, b, c);
+            v0 = T((double)v0 * d); v1 = T((double)v1 * d);
+        }
+        tile[rr * KR16_LD + kr16_pad(c)] = v0;
+        tile[(rr + 1) * KR16_LD + kr16_pad(c)] = v1;
+    }
+    __syncthreads();
+    T ef[4], ef2[4];
+    load_efrag<T>(a.E, lane, ef, ef2);
+#pragma unroll
+    for (int q = 0; q < KR16_ROWS / 8; ++q) {
+        T* row = tile + (wave + 8 * q) * KR16_LD;
+        T vin[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) vin[s] = row[kr16_pad(lane + 64 * s)];
+        typename Mfma<T>::acc_t z;
+        kron16_vec<T>(ef, ef2, vin, z);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) row[kr16_pad(Mfma<T>::row(lane, r) + 16 * (lane & 15))] = z[r];
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int e = tid; e < KR16_ROWS / 2 * 256; e += 512) {
+        const int rr = 2 * (e & 15), c = e >> 4;
+        double v0 = (double)tile[rr * KR16_LD + kr16_pad(c)], v1 = (double)tile[(rr + 1) * KR16_LD + kr16_pad(c)];
+        if (a.rowd.mode != DIAG_NONE) {
+            v0 *= diag_value(a.rowd, b, r0 + rr);
+            v1 *= diag_value(a.rowd, b, r0 + rr + 1);
+        }
+        if (a.cold.mode != DIAG_NONE) {
+            const double d = diag_value(a.cold, b, c);
+            v0 *= d; v1 *= d;
+        }
+        T* dst = C + r0 + rr + (long long)c * 256;
+        if (sizeof(T) == 8) {
+            double2 v; v.x = v0; v.y = v1;
+            *reinterpret_cast<double2*>(dst) = v;
+        } else {
+            float2 v; v.x = (float)v0; v.y = (float)v1;
+            *reinterpret_cast<float2*>(dst) = v;
+        }
+    }
+}
+
+template <typename T>
+hipError_t launch_kron_left(const KronArgs<T>& a, hipStream_t s)
+{
+    if (a.nl > 16) return hipErrorInvalidValue;
+    dim3 grid((a.N + 15) / 16, a.batch);
+    if (a.nl == 16)
+        hipLaunchKernelGGL(kron_left16_kernel<T>, grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(kron_left_kernel<T>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_kron_right(const KronArgs<T>& a, hipStream_t s)
+{
+    if (a.nl > 16) return hipErrorInvalidValue;
+    dim3 grid((a.N + 15) / 16, a.batch);
+    if (a.nl == 16)
+        hipLaunchKernelGGL(kron_right16_kernel<T>, dim3(256 / KR16_ROWS, a.batch), dim3(512), 0, s, a);
+    else
+        hipLaunchKernelGGL(kron_right_kernel<T>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+// exp(x A1) for the periodic ring of nl sites (A1[i][i+1] += 1, A1[i+1][i] += 1, so nl = 2 gives
+// 2 and nl = 1 gives 2 on the diagonal), padded to a 16 x 16 row-major block.  The square
+// lattice's exp(x A) is its Kronecker square.  Host code; rings of more than 16 sites have no
+// padded factor (their products are dense GEMMs) and leave out zero.
+void ring_expk16(int nl, double x, double* out)
+{
+    for (int e = 0; e < 256; ++e) out[e] = 0.0;
+    if (nl > 16) return;
+    if (x == 0.0) {   // exactly the identity (the cosine sums below round to ~1e-17)
+        for (int i = 0; i < nl; ++i) out[i * 16 + i] = 1.0;
+        return;
+    }
+    for (int i = 0; i < nl; ++i)
+        for (int j = 0; j < nl; ++j) {
+            double s = 0.0;
+            for (int m = 0; m < nl; ++m) {
+                const double k = 2.0 * M_PI * m / nl;
+                s += std::cos(k * (i - j)) * std::exp(2.0 * x * std::cos(k));
+            }
+            out[i * 16 + j] = s / nl;
+        }
+}
+
+template hipError_t launch_kron_left<double>(const KronArgs<double>&, hipStream_t);
+template hipError_t launch_kron_right<double>(const KronArgs<double>&, hipStream_t);
+template hipError_t launch_kron_left<float>(const KronArgs<float>&, hipStream_t);
+template hipError_t launch_kron_right<float>(const KronArgs<float>&, hipStream_t);
+
+}  // namespace dqmc

@@ -109,9 +109,10 @@ hipError_t prepare_kernels();
 // applying Q^T to the trailing W-N columns.  On exit the leading block holds R (zeros
 // below the diagonal).  One-kernel form (DQMC_QR=v1).
 hipError_t launch_qr_v1(double* buf, long long stride, int N, int W, int batch, hipStream_t s);
-// Blocked Householder QR of the same row-major N x W system (N <= 256): per panel of 32 columns
+// Blocked Householder QR of the same row-major N x W system (N <= 1024): per panel of 32 columns
 // one factorization launch (one workgroup per chain) and the trailing update of the remaining
-// columns.  Scratch per chain: Vs N*32 doubles, Ts 32*32 doubles, and Wt holding two blocks of
+// columns.  Systems of more than 256 rows use the wide panel kernel (8 waves, up to 2 rows per
+// thread) and the three-GEMM update in order, whatever `update`, `panel` and `la` ask for.  Scratch per chain: Vs N*32 doubles, Ts 32*32 doubles, and Wt holding two blocks of
 // wt_stride >= W*32 doubles (batch * wt_stride apart).
 // `update`: QR_UPDATE_REG (default) applies the whole block reflector in one launch per panel with
 // each 32-column tile held in registers (the first Wt block holds a row-major copy of V);

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(256) void meas_dyn_kernel(const double* __restrict_
                                                        long long sv, int L, int nl, const double* __restrict__ costab,
                                                        int nk, double* __restrict__ cache, int cstride, int flags)
 {
-    __shared__ double S[256];
+    __shared__ double S[1024];
     __shared__ double red[4];
     const int t = blockIdx.x, b = blockIdx.y, tid = threadIdx.x;
     const int N = nl * nl;
@@ -151,7 +151,7 @@ hipError_t launch_measure_dynamic(const double* gt0, const double* g0t, long lon
                                   const double* costab, int nk, double* cache, int cstride, int flags, int batch,
                                   hipStream_t s)
 {
-    if (nl * nl > 256) return hipErrorInvalidValue;
+    if (nl * nl > 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL(meas_dyn_kernel, dim3(L, batch), dim3(256), 0, s, gt0, g0t, sv, L, nl, costab, nk, cache,
                        cstride, flags);
     return hipGetLastError();

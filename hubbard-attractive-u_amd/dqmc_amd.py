@@ -122,9 +122,9 @@ class Engine:
     """A batch of `n_chains` DQMC chains on one GPU."""
 
     def __init__(self, nl, nt, beta, u, mu=0.0, nnt=1.0, pace=10, n_chains=1, precision=FP64, delay_k=1,
-                 eqtime=False, dynamic=False):
+                 eqtime=False, dynamic=False, is_fft=False):
         self.nl, self.n, self.nt, self.n_chains = nl, nl * nl, nt, n_chains
-        p = Params(nl, nt, beta, nnt, u, mu, pace, 0, int(eqtime), int(dynamic))
+        p = Params(nl, nt, beta, nnt, u, mu, pace, int(is_fft), int(eqtime), int(dynamic))
         h = C.c_void_p()
         rc = lib().dqmc_create(C.byref(p), n_chains, precision, delay_k, C.byref(h))
         self._h = h

@@ -38,7 +38,9 @@ typedef struct dqmc_params_c {
     double u;                /* [Model.HubbardAttractiveU] u (absolute value of U)   */
     double mu;               /* [Model.HubbardAttractiveU] mu (default of each chain)*/
     int stabilization_pace;  /* [MonteCarlo] stabilization_pace                      */
-    int is_fft;              /* [MonteCarlo] is_fft: accepted, the dense path is used */
+    int is_fft;              /* [MonteCarlo] is_fft: accepted; both values apply the same
+                              * exp(-+dt K), built from the lattice dispersion (the diagonal
+                              * form the reference's FFT path computes, src/hubbard.cpp:50-81) */
     int collect_eqtime;      /* keep G(t,t) of every slice (Measurement::Handle::isEqualTime) */
     int collect_dynamic;     /* keep G(t,0), G(0,t) (Measurement::Handle::isDynamic)  */
 } dqmc_params_c;
@@ -61,7 +63,9 @@ enum {
  * on the block of G, and the block's accepted flips are applied to G as one rank-|A| update.
  * delay_k = 1 applies every accepted flip immediately, the reference's rank-1
  * Sherman-Morrison update (src/hubbard.cpp:132-139).  The accept/reject sequence does not
- * depend on delay_k.
+ * depend on delay_k.  Lattices: 1 <= nl <= 32 (N <= 1024).  Above 16 x 16 the pending update
+ * of a block is limited by LDS, so delay_k is capped (FP64: 24 at nl = 18, 12 at nl = 24, 8 at
+ * nl = 32) and the slice products are dense MFMA GEMMs.
  * Replaces: construction of DQMC::Core + Model::HubbardAttractiveU + SquareLattice and their
  * initialize() calls (src/main.cpp:119-128; src/dqmc_core.cpp:20-30; src/hubbard.cpp:25-48). */
 int dqmc_create(const dqmc_params_c* params, int n_chains, int precision, int delay_k, dqmc_ctx** out);

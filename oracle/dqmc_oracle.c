@@ -11,7 +11,8 @@
  *   RNG ............ src/random.cpp:14-15, libstdc++ minstd_rand0 +
  *                    generate_canonical<double,53> + bernoulli_distribution
  *   lattice / K .... src/square_lattice.cpp:30-61, src/hubbard.cpp:25-48
- *   B operations ... src/hubbard.cpp:148-156, 254-341
+ *   B operations ... src/hubbard.cpp:148-156, 254-341; the is_fft products
+ *                    src/hubbard.cpp:50-81, 159-246 (orc_set_fft)
  *   ratio / SM ..... src/hubbard.cpp:123-139
  *   SVD stack ...... include/utils/svd_stack.hpp:51-188
  *   numerics ....... include/utils/stable_numerics.hpp:38-324
@@ -31,10 +32,17 @@
  *     column below the diagonal.
  *   - exp(-dt K): K is real symmetric; K = W diag(l) W^T by cyclic Jacobi, then
  *     exp(-dt K) = W diag(exp(-dt l)) W^T.
+ *   - is_fft (MKL DFTI, fft_solver.hpp:20-91): a direct 2-D DFT of length nl per
+ *     axis, with the full complex spectrum.  The reference asks MKL for a real
+ *     transform in the conjugate-even packed layout (fft_solver.hpp:60-61), which
+ *     holds only nl x (nl/2+1) frequencies; the restatement keeps the operator
+ *     the reference's comments state, (inv)expK = F^-1 diag(e^{-+dt E(k)}) F
+ *     (src/hubbard.cpp:174-178).
  * When MKL's runtime (libmkl_rt) can be dlopen'ed, orc_set_mkl(1) routes the
  * GEMMs and the SVD through cblas_dgemm / LAPACKE_dgesvd exactly as the
- * reference does (EIGEN_USE_MKL_ALL, linear_algebra.hpp:74); this is used only
- * to time a realistic CPU baseline.
+ * reference does (EIGEN_USE_MKL_ALL, linear_algebra.hpp:74), and the
+ * eigendecomposition behind exp(-dt K) through LAPACKE_dsyev; this times a
+ * realistic CPU baseline and keeps the checks on large lattices fast.
  */
 #include <dlfcn.h>
 #include <float.h>
@@ -53,9 +61,11 @@ typedef void (*cblas_dgemm_t)(int, int, int, int, int, int, double, const double
                               const double*, int, double, double*, int);
 typedef int (*lapacke_dgesvd_t)(int, char, char, int, int, double*, int, double*, double*, int,
                                 double*, int, double*);
+typedef int (*lapacke_dsyev_t)(int, char, char, int, double*, int, double*);
 typedef int (*mkl_set_layer_t)(int);
 static cblas_dgemm_t p_dgemm = NULL;
 static lapacke_dgesvd_t p_dgesvd = NULL;
+static lapacke_dsyev_t p_dsyev = NULL;
 static int g_use_mkl = 0;
 
 int orc_set_mkl(int on)
@@ -69,6 +79,7 @@ int orc_set_mkl(int on)
         if (setl) setl(1 /* MKL_THREADING_SEQUENTIAL */);
         p_dgemm = (cblas_dgemm_t)dlsym(h, "cblas_dgemm");
         p_dgesvd = (lapacke_dgesvd_t)dlsym(h, "LAPACKE_dgesvd");
+        p_dsyev = (lapacke_dsyev_t)dlsym(h, "LAPACKE_dsyev");
         if (!p_dgemm || !p_dgesvd) { p_dgemm = NULL; p_dgesvd = NULL; return -2; }
     }
     g_use_mkl = 1;
@@ -206,21 +217,36 @@ static void sym_eig_jacobi(int n, double* a /* destroyed */, double* w, double* 
     for (int i = 0; i < n; ++i) w[i] = a[IDX(i, i, n)];
 }
 
-/* out = exp(scale * K) for symmetric K */
-static void sym_expm(int n, const double* K, double scale, double* out)
+/* K = v diag(w) v^T for symmetric K: LAPACKE_dsyev when MKL is loaded (large lattices), else the
+ * cyclic Jacobi iteration above */
+static void sym_eig(int n, const double* K, double* w, double* v)
 {
+    if (g_use_mkl && p_dsyev) {
+        mat_copy(n, v, K);
+        if (p_dsyev(102 /*ColMajor*/, 'V', 'U', n, v, n, w) == 0) return;
+    }
     double* a = mat_alloc(n);
-    double* v = mat_alloc(n);
-    double* w = (double*)malloc(sizeof(double) * n);
     mat_copy(n, a, K);
     sym_eig_jacobi(n, a, w, v);
+    free(a);
+}
+
+/* out = v diag(exp(scale w)) v^T, each entry summed over k in order */
+static void sym_expm_from(int n, const double* w, const double* v, double scale, double* out)
+{
+    double* e = (double*)malloc(sizeof(double) * n);
+    double* vr = mat_alloc(n);   /* row-major copy: vr[i n + k] = v(i, k) */
+    for (int k = 0; k < n; ++k) e[k] = exp(scale * w[k]);
+    mat_transpose(n, vr, v);
     for (int j = 0; j < n; ++j)
         for (int i = 0; i < n; ++i) {
+            const double* vi = vr + (size_t)i * n;
+            const double* vj = vr + (size_t)j * n;
             double s = 0.0;
-            for (int k = 0; k < n; ++k) s += v[IDX(i, k, n)] * exp(scale * w[k]) * v[IDX(j, k, n)];
+            for (int k = 0; k < n; ++k) s += vi[k] * e[k] * vj[k];
             out[IDX(i, j, n)] = s;
         }
-    free(a); free(v); free(w);
+    free(e); free(vr);
 }
 
 /* exported for tests: nearest-neighbour adjacency of the periodic nl x nl lattice
@@ -246,9 +272,12 @@ void orc_expk(int nl, double dt, double nnt, double mu, double* expK, double* in
     orc_nn_hoppings(nl, K);
     for (size_t e = 0; e < (size_t)n * n; ++e) K[e] *= -nnt;
     for (int i = 0; i < n; ++i) K[IDX(i, i, n)] -= mu;
-    if (expK) sym_expm(n, K, -dt, expK);
-    if (invExpK) sym_expm(n, K, +dt, invExpK);
-    free(K);
+    double* v = mat_alloc(n);
+    double* w = (double*)malloc(sizeof(double) * n);
+    sym_eig(n, K, w, v);
+    if (expK) sym_expm_from(n, w, v, -dt, expK);
+    if (invExpK) sym_expm_from(n, w, v, +dt, invExpK);
+    free(K); free(v); free(w);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -673,6 +702,10 @@ typedef struct {
     double wrap_eq, wrap_dyn;
     long long accepts, visits;
     uint32_t* accmask;  /* [t][(n+31)/32]: accept bits of the last updating sweep */
+    /* is_fft products: eigenvalues e^{-+dt E(k)} of (inv)expK at k = kx + nl*ky, twiddles
+     * cos/sin(2 pi m / nl) at m, and complex work columns */
+    int is_fft;
+    double *lam_f, *lam_i, *tw_c, *tw_s, *wre, *wim, *vre, *vim;
 } orc_ctx;
 
 orc_ctx* orc_create(int nl, int nt, double beta, double nnt, double u, double mu, int pace, int eqtime, int dynamic)
@@ -710,6 +743,7 @@ void orc_destroy(orc_ctx* c)
     free(c->gftt); free(c->gft0); free(c->gf0t); free(c->stacks); free(c->t1); free(c->t2); free(c->t3); free(c->t4);
     free(c->col); free(c->row);
     free(c->vecgftt); free(c->vecgft0); free(c->vecgf0t); free(c->accmask);
+    free(c->lam_f); free(c->lam_i); free(c->tw_c); free(c->tw_s); free(c->wre); free(c->wim); free(c->vre); free(c->vim);
     free(c);
 }
 
@@ -746,12 +780,114 @@ static void scale_cols(orc_ctx* c, double* g, int t, double sgn)
         for (int i = 0; i < n; ++i) g[IDX(i, j, n)] *= f;
     }
 }
+/* ------------------------------------------------------------------------- */
+/*  is_fft products (src/hubbard.cpp:50-81, :159-246): the translation-        */
+/*  invariant K is diagonal in momentum space, (inv)expK = F^-1 diag(e^{-+dt   */
+/*  E(k)}) F with F the 2-D DFT over the site index i = x + nl*y, applied      */
+/*  column by column from the left; a product from the right transposes, so    */
+/*  X expK = (expK X^T)^T (expK is symmetric).  The eigenvalues are the DFT of */
+/*  K's column 0, E(k) = sum_d K(d, 0) e^{-2 pi i k.d / nl}: the diagonal of F */
+/*  K F^-1 that src/hubbard.cpp:63-76 forms with two batches of transforms.    */
+/* ------------------------------------------------------------------------- */
+void orc_fft_eigens(int nl, double dt, double nnt, double mu, double* lam_f, double* lam_i)
+{
+    const int n = nl * nl;
+    double* K = mat_alloc(n);
+    orc_nn_hoppings(nl, K);
+    for (size_t e = 0; e < (size_t)n * n; ++e) K[e] *= -nnt;
+    for (int i = 0; i < n; ++i) K[IDX(i, i, n)] -= mu;
+    for (int ky = 0; ky < nl; ++ky)
+        for (int kx = 0; kx < nl; ++kx) {
+            double e = 0.0;
+            for (int d = 0; d < n; ++d) {
+                const int m = (kx * (d % nl) + ky * (d / nl)) % nl;
+                e += K[IDX(d, 0, n)] * cos(2.0 * M_PI * m / nl);
+            }
+            if (lam_f) lam_f[kx + nl * ky] = exp(-dt * e);
+            if (lam_i) lam_i[kx + nl * ky] = exp(+dt * e);
+        }
+    free(K);
+}
+
+/* one DFT pass of length nl along x (stride 1) or y (stride nl) of the complex column (re, im) into
+ * (ore, oim); sign -1 forward, +1 backward (unscaled) */
+static void dft_pass(const orc_ctx* c, const double* re, const double* im, double* ore, double* oim, int along_y,
+                     double sign)
+{
+    const int nl = c->nl;
+    for (int a = 0; a < nl; ++a)          /* the other coordinate */
+        for (int k = 0; k < nl; ++k) {    /* output frequency (or position) */
+            double sr = 0.0, si = 0.0;
+            for (int m = 0; m < nl; ++m) {
+                const int idx = along_y ? a + nl * m : m + nl * a;
+                const int w = (k * m) % nl;
+                const double cr = c->tw_c[w], ci = sign * c->tw_s[w];
+                sr += re[idx] * cr - im[idx] * ci;
+                si += re[idx] * ci + im[idx] * cr;
+            }
+            const int o = along_y ? a + nl * k : k + nl * a;
+            ore[o] = sr;
+            oim[o] = si;
+        }
+}
+
+/* g <- (inv)expK g through the DFT, lam = lam_f or lam_i (fft_mult_from_left, src/hubbard.cpp:172-189) */
+static void fft_left(orc_ctx* c, double* g, const double* lam)
+{
+    const int n = c->n;
+    for (int j = 0; j < n; ++j) {
+        double* v = g + (size_t)j * n;
+        for (int i = 0; i < n; ++i) { c->vre[i] = v[i]; c->vim[i] = 0.0; }
+        dft_pass(c, c->vre, c->vim, c->wre, c->wim, 0, -1.0);
+        dft_pass(c, c->wre, c->wim, c->vre, c->vim, 1, -1.0);
+        for (int k = 0; k < n; ++k) { c->vre[k] *= lam[k]; c->vim[k] *= lam[k]; }
+        dft_pass(c, c->vre, c->vim, c->wre, c->wim, 1, +1.0);
+        dft_pass(c, c->wre, c->wim, c->vre, c->vim, 0, +1.0);
+        for (int i = 0; i < n; ++i) v[i] = c->vre[i] / n;
+    }
+}
+/* g <- g (inv)expK: transpose, from the left, transpose back (src/hubbard.cpp:218-230) */
+static void fft_right(orc_ctx* c, double* g, const double* lam)
+{
+    mat_transpose(c->n, c->t3, g);
+    fft_left(c, c->t3, lam);
+    mat_transpose(c->n, g, c->t3);
+}
+
+/* switch the expK products of a chain to the is_fft form (call before orc_init) */
+void orc_set_fft(orc_ctx* c, int on)
+{
+    c->is_fft = on;
+    if (!on || c->lam_f) return;
+    const int n = c->n, nl = c->nl;
+    c->lam_f = (double*)malloc(sizeof(double) * n); c->lam_i = (double*)malloc(sizeof(double) * n);
+    c->tw_c = (double*)malloc(sizeof(double) * nl); c->tw_s = (double*)malloc(sizeof(double) * nl);
+    c->wre = (double*)malloc(sizeof(double) * n); c->wim = (double*)malloc(sizeof(double) * n);
+    c->vre = (double*)malloc(sizeof(double) * n); c->vim = (double*)malloc(sizeof(double) * n);
+    for (int m = 0; m < nl; ++m) { c->tw_c[m] = cos(2.0 * M_PI * m / nl); c->tw_s[m] = sin(2.0 * M_PI * m / nl); }
+    orc_fft_eigens(nl, c->dt, c->nnt, c->mu, c->lam_f, c->lam_i);
+}
+
+/* expK products of either form: fwd selects expK (else invExpK), left from the left (else right) */
+static void expk_mul(orc_ctx* c, double* g, int fwd, int left)
+{
+    if (c->is_fft) {
+        if (left) fft_left(c, g, fwd ? c->lam_f : c->lam_i);
+        else fft_right(c, g, fwd ? c->lam_f : c->lam_i);
+        return;
+    }
+    const double* E = fwd ? c->expK : c->invExpK;
+    if (left) gemm(c->n, 0, E, 0, g, c->t3);
+    else gemm(c->n, 0, g, 0, E, c->t3);
+    mat_copy(c->n, g, c->t3);
+}
+
 /* B-ops (src/hubbard.cpp:293-341); the expK GEMMs are src/hubbard.cpp:148-156 */
-static void mul_B_left(orc_ctx* c, double* g, int t)     { gemm(c->n, 0, c->expK, 0, g, c->t3); mat_copy(c->n, g, c->t3); scale_rows(c, g, t, +1); }
-static void mul_B_right(orc_ctx* c, double* g, int t)    { scale_cols(c, g, t, +1); gemm(c->n, 0, g, 0, c->expK, c->t3); mat_copy(c->n, g, c->t3); }
-static void mul_invB_left(orc_ctx* c, double* g, int t)  { scale_rows(c, g, t, -1); gemm(c->n, 0, c->invExpK, 0, g, c->t3); mat_copy(c->n, g, c->t3); }
-static void mul_invB_right(orc_ctx* c, double* g, int t) { gemm(c->n, 0, g, 0, c->invExpK, c->t3); mat_copy(c->n, g, c->t3); scale_cols(c, g, t, -1); }
-static void mul_adjB_left(orc_ctx* c, double* g, int t)  { scale_rows(c, g, t, +1); gemm(c->n, 0, c->expK, 0, g, c->t3); mat_copy(c->n, g, c->t3); }
+static void mul_B_left(orc_ctx* c, double* g, int t)     { expk_mul(c, g, 1, 1); scale_rows(c, g, t, +1); }
+static void mul_B_right(orc_ctx* c, double* g, int t)    { scale_cols(c, g, t, +1); expk_mul(c, g, 1, 0); }
+static void mul_invB_left(orc_ctx* c, double* g, int t)  { scale_rows(c, g, t, -1); expk_mul(c, g, 0, 1); }
+static void mul_invB_right(orc_ctx* c, double* g, int t) { expk_mul(c, g, 0, 0); scale_cols(c, g, t, -1); }
+static void mul_adjB_left(orc_ctx* c, double* g, int t)  { scale_rows(c, g, t, +1); expk_mul(c, g, 1, 1); }
 
 /* exported B-op for unit tests: op 0..4 = B_left, B_right, invB_left, invB_right, adjB_left */
 void orc_apply_bop(orc_ctx* c, int op, int t, double* g)

@@ -40,6 +40,8 @@ def lib():
             "orc_rng_canonical_stream": (None, [C.c_uint64, C.c_int, _dp]),
             "orc_nn_hoppings": (None, [C.c_int, _dp]),
             "orc_expk": (None, [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]),
+            "orc_fft_eigens": (None, [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]),
+            "orc_set_fft": (None, [C.c_void_p, C.c_int]),
             "orc_svd": (C.c_int, [C.c_int, _dp, _dp, _dp, _dp]),
             "orc_inverse": (C.c_int, [C.c_int, _dp, _dp]),
             "orc_fullpiv_qr_solve": (None, [C.c_int, _dp, _dp, _dp]),
@@ -120,6 +122,14 @@ def expk(nl, dt, nnt, mu):
     return e, ie
 
 
+def fft_eigens(nl, dt, nnt, mu):
+    """e^{-dt E(k)} and e^{+dt E(k)} at k = kx + nl*ky: the diagonal of (inv)expK in momentum space."""
+    n = nl * nl
+    lf, li = np.zeros(n), np.zeros(n)
+    lib().orc_fft_eigens(nl, dt, nnt, mu, _ptr(lf), _ptr(li))
+    return lf, li
+
+
 def svd(a):
     a = np.asfortranarray(a, dtype=np.float64)
     n = a.shape[0]
@@ -155,10 +165,12 @@ def fullpiv_qr_solve(h, b):
 class Chain:
     """One reference Markov chain: DQMC::Core + Model::HubbardAttractiveU restated (FP64, CPU)."""
 
-    def __init__(self, nl, nt, beta, u, mu, nnt=1.0, pace=10, eqtime=False, dynamic=False):
+    def __init__(self, nl, nt, beta, u, mu, nnt=1.0, pace=10, eqtime=False, dynamic=False, is_fft=False):
         self.nl, self.n, self.nt = nl, nl * nl, nt
         self.beta, self.u, self.mu, self.nnt, self.pace = beta, u, mu, nnt, pace
         self._c = lib().orc_create(nl, nt, beta, nnt, u, mu, pace, int(eqtime), int(dynamic))
+        if is_fft:
+            lib().orc_set_fft(self._c, 1)
 
     def close(self):
         if self._c:
