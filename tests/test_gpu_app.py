@@ -163,3 +163,54 @@ def test_app_statistics_agree_with_reference_example(tmp_path):
             say("  %s[%d]: %.6f +- %.6f vs reference %.6f +- %.6f (z=%.2f)" %
                 (stem, i, mine[i, 0], mine[i, 2], ref[i, 0], ref[i, 2], z))
     assert worst < 5.0
+
+
+def read_fields(path, nt, ns):
+    lines = open(path).read().split("\n")
+    assert lines[0].split() == [str(nt), str(ns)]
+    s = np.zeros((nt, ns), dtype=np.int32)
+    for l in lines[1:]:
+        p = l.split()
+        if len(p) == 3:
+            s[int(p[0]), int(p[1])] = int(p[2])
+    return s
+
+
+def test_app_fields_file_input_follows_oracle(tmp_path):
+    # `-f ising.fields` (src/main.cpp:131-135): the reference's own example fields (nl=4, nt=160) as
+    # the starting configuration; no random draws for the fields, so the Metropolis stream starts at
+    # the seed.  The app writes the final configuration back to the same file.
+    say("case: app -f example/ising.fields, 4x4 nt=160 beta=8")
+    nl, nt, beta, u, mu, pace, seed = 4, 160, 8.0, 4.0, 0.0, 10, 2024
+    fpath = tmp_path / "ising.fields"
+    fpath.write_bytes(open(os.path.join(EX, "ising.fields"), "rb").read())
+    start = read_fields(fpath, nt, nl * nl)
+    cfg = tmp_path / "f.toml"
+    eq = ["FillingNumber", "DoubleOccupation", "StaticSWavePairingCorrelation"]
+    config(cfg, nl, nt, beta, u, mu, pace, 4, 2, 3, 0, eq, 1, seed, 8)
+    out = tmp_path / "out"
+    r = subprocess.run([APP, "-c", str(cfg), "-o", str(out), "-f", str(fpath)], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "loaded from file" in r.stdout
+    o = orc.Chain(nl, nt, beta, u, mu, pace=pace, eqtime=True)
+    o.set_seed(seed)
+    o.fields = start
+    assert o.init() == 0
+    o.set_thermalization(True)
+    for _ in range(2):
+        o.sweep_up(); o.sweep_down()
+    o.set_thermalization(False)
+    want = {k: [] for k in eq}
+    for b in range(2):
+        acc = np.zeros(3)
+        for s in range(3):
+            o.sweep_up(); acc += o.measure_equaltime()
+            o.sweep_down(); acc += o.measure_equaltime()
+        for k, v in zip(eq, acc / 6):
+            want[k].append(v)
+    for k, stem in zip(eq, ["filling", "double_occu", "swave_corr"]):
+        d = np.abs(bins(out / (stem + ".bins.out"), 0) - np.array(want[k])).max()
+        say("  %s max |bin - oracle| = %g" % (k, d))
+        assert d < 1e-9
+    assert np.array_equal(read_fields(fpath, nt, nl * nl), o.fields)
