@@ -394,6 +394,8 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
     // B+1's rebuild reads after the next barrier).
     __syncthreads();
     uint32_t* health = a.health + b;
+    // wave 0 runs the latency-bound site loops beside the flushing waves: it issues first
+    if (tid < 64) __builtin_amdgcn_s_setprio(3);
     if (tid < 64) metro_site_block(L, G, f, N, kb, 0, min(kb, N), dpos, dneg, health);
     __syncthreads();
     const int lane = tid & 63, wave = tid >> 6, nwv = nth >> 6, l16 = lane & 15;

@@ -162,6 +162,8 @@ __global__ __launch_bounds__(1024) void qr_panel4_kernel(double* __restrict__ bu
                                                          double* __restrict__ Ts, double* __restrict__ Vrows,
                                                          long long vr_stride)
 {
+    // latency-bound: issue ahead of the update kernels that share the CU
+    __builtin_amdgcn_s_setprio(3);
     __shared__ double red[16][QCPT];        // per-wave partial dot products
     __shared__ double red1[16];             // per-wave partial norms
     __shared__ double xcol[256];            // column i of the panel (before its reflection)
@@ -346,6 +348,35 @@ __device__ __forceinline__ int halve32_index(int lane)
            ((lane & 2) ? 1 : 0);
 }
 
+// Row-major write-back of a 256-thread panel held one row per thread (thread r: row r, x[0..QRB-1])
+// through an LDS transpose in two halves of 16 columns: every store instruction then writes whole
+// 128-byte row segments (4 rows x 16 columns per wave) instead of 64 scattered doubles.  Row r,
+// column m of the panel goes to dst0[r * ld0 + m] as f0(r, m, x) and, when dst1 is set, to
+// dst1[r * ld1 + m] as f1(r, m, x), for r < R and m < nb.  All 256 threads must call it.
+template <typename F0, typename F1>
+__device__ __forceinline__ void panel_rows_store(double (*tp)[QRB / 2 + 1], const double (&x)[QRB], int R, int nb,
+                                                 double* __restrict__ dst0, long long ld0, F0 f0,
+                                                 double* __restrict__ dst1, long long ld1, F1 f1)
+{
+    const int r = threadIdx.x;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (h) __syncthreads();   // the first half's reads are done before its slots are reused
+#pragma unroll
+        for (int j = 0; j < QRB / 2; ++j) tp[r][j] = x[QRB / 2 * h + j];
+        __syncthreads();
+#pragma unroll 4
+        for (int e = r; e < 256 * (QRB / 2); e += 256) {
+            const int row = e >> 4, m = QRB / 2 * h + (e & 15);
+            if (row < R && m < nb) {
+                const double v = tp[row][e & 15];
+                dst0[row * ld0 + m] = f0(row, m, v);
+                if (dst1) dst1[row * ld1 + m] = f1(row, m, v);
+            }
+        }
+    }
+}
+
 // Panel factorization, 256 threads (4 waves) per chain: thread r holds panel row r (rows p0 .. N-1,
 // so N - p0 <= 256) with all QRB columns in registers.  Column i costs two 4-wave barriers: one
 // after the partial norms (alpha = x(i,i) is published with them), one after the partial sums of
@@ -360,11 +391,14 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs
                                                        double* __restrict__ Ts, double* __restrict__ Vrows,
                                                        long long vr_stride)
 {
+    // latency-bound: issue ahead of the update kernels that share the CU
+    __builtin_amdgcn_s_setprio(3);
     __shared__ double nrm[4];
     __shared__ double alph;
     __shared__ double dot[4][QRB];
     __shared__ __align__(16) double VtVt[QRB][QRB];   // VtVt[i][k] = (V^T V)(k, i), k < i
     __shared__ double tau_s[QRB];
+    __shared__ double tp[256][QRB / 2 + 1];            // write-back transpose
     const int r = threadIdx.x, lane = r & 63, wave = r >> 6;
     const int b = blockIdx.x;
     const int R = N - p0;
@@ -418,19 +452,14 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs
     __syncthreads();
     if (live) {
 #pragma unroll
-        for (int m = 0; m < QRB; ++m) {
-            if (m < nb) {
-                A[(long long)(p0 + r) * W + p0 + m] = (r <= m) ? x[m] : 0.0;
-                V[r + (long long)m * N] = (r == m) ? 1.0 : (r > m ? x[m] : 0.0);
-            }
-        }
-        if (Vrows) {
-            double* vr = Vrows + (long long)b * vr_stride + (long long)r * QRB;
-#pragma unroll
-            for (int m = 0; m < QRB; ++m)
-                if (m < nb) vr[m] = (r == m) ? 1.0 : (r > m ? x[m] : 0.0);
-        }
+        for (int m = 0; m < QRB; ++m)
+            if (m < nb) V[r + (long long)m * N] = (r == m) ? 1.0 : (r > m ? x[m] : 0.0);
     }
+    // R back into the system and the row-major copy of V (rows at QRB), as whole row segments
+    panel_rows_store(tp, x, R, nb, A + (long long)p0 * W + p0, W,
+                     [](int row, int m, double v) { return row <= m ? v : 0.0; },
+                     Vrows ? Vrows + (long long)b * vr_stride : nullptr, QRB,
+                     [](int row, int m, double v) { return row == m ? 1.0 : (row > m ? v : 0.0); });
     if (r < QRB) {
         // row j of T: T(j, k) = 0 for k < j, so the sums run over all k < i without a guard
         // (tau_i = 0 for i >= nb leaves those columns zero)
@@ -589,6 +618,8 @@ __global__ __launch_bounds__(64 * NW) void qr_panel_wide_kernel(double* __restri
                                                                 int W, int p0, int nb, double* __restrict__ Vs,
                                                                 double* __restrict__ Ts)
 {
+    // latency-bound: issue ahead of the update kernels that share the CU
+    __builtin_amdgcn_s_setprio(3);
     constexpr int NT = 64 * NW;
     __shared__ double nrm[NW];
     __shared__ double alph;
@@ -874,6 +905,8 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs
                                                        double* __restrict__ Li, int* __restrict__ Ps,
                                                        uint32_t* __restrict__ health)
 {
+    // latency-bound: issue ahead of the update kernels that share the CU
+    __builtin_amdgcn_s_setprio(3);
     __shared__ double redv[2][4];
     __shared__ int redi[2][4];
     __shared__ __align__(16) double cand[2][4][QRB];
@@ -881,6 +914,7 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs
     __shared__ __align__(16) double irow[2][QRB];
     __shared__ int irowo[2];
     __shared__ double L11[QRB][QRB + 1];
+    __shared__ double tp[256][QRB / 2 + 1];            // write-back transpose
     const int r = threadIdx.x, lane = r & 63, wave = r >> 6;
     const int b = blockIdx.x;
     const int R = N - p0;
@@ -949,12 +983,12 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs
     if (live) {
 #pragma unroll
         for (int m = 0; m < QRB; ++m)
-            if (m < nb) {
-                A[(long long)(p0 + r) * W + p0 + m] = x[m];
-                Ls[(long long)b * N * QRB + r + (long long)m * N] = (r > m) ? x[m] : (r == m ? 1.0 : 0.0);
-            }
+            if (m < nb) Ls[(long long)b * N * QRB + r + (long long)m * N] = (r > m) ? x[m] : (r == m ? 1.0 : 0.0);
         Ps[(long long)b * N + r] = org;
     }
+    // L and U back into the system, as whole row segments
+    panel_rows_store(tp, x, R, nb, A + (long long)p0 * W + p0, W, [](int, int, double v) { return v; },
+                     (double*)nullptr, 0, [](int, int, double v) { return v; });
     if (r < QRB) {
 #pragma unroll
         for (int m = 0; m < QRB; ++m) L11[r][m] = (r > m && r < nb && m < nb) ? x[m] : 0.0;

@@ -187,7 +187,9 @@ def test_app_fields_file_input_follows_oracle(tmp_path):
     start = read_fields(fpath, nt, nl * nl)
     cfg = tmp_path / "f.toml"
     eq = ["FillingNumber", "DoubleOccupation", "StaticSWavePairingCorrelation"]
-    config(cfg, nl, nt, beta, u, mu, pace, 4, 2, 3, 0, eq, 1, seed, 8)
+    # equal-time only: a bin of capacity 6 is ceil(6/2) = 3 up/down pairs, measured after each sweep
+    # (src/dqmc_handle.cpp:98, 125-146)
+    config(cfg, nl, nt, beta, u, mu, pace, 4, 2, 6, 0, eq, 1, seed, 8)
     out = tmp_path / "out"
     r = subprocess.run([APP, "-c", str(cfg), "-o", str(out), "-f", str(fpath)], capture_output=True, text=True,
                        timeout=300)
