@@ -3,6 +3,8 @@
 // hopping exponential on MFMA.
 #include "dqmc_device.h"
 
+#include <type_traits>
+
 namespace dqmc {
 
 // ---------------------------------------------------------------------------------------------
@@ -239,6 +241,129 @@ __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs<T> g)
             }
 }
 
+// FP64 128 x 128 output tile: 4 waves in a 2 x 2 grid, each a 64 x 64 block of 4 x 4 MFMA tiles,
+// so every LDS fragment feeds four MFMAs and every global byte eight times the flops of the 64 x 64
+// tile.  K advances by 8 through double-buffered LDS, with the global loads of the next step in one
+// register stage, so that two workgroups fit a CU.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm128_kernel(GemmArgs<double> g)
+{
+    constexpr int BM = 128, BN = 128, BK = 8, WM = 64, WN = 64, NA = BM * BK / 256, NB = BN * BK / 256;
+    __shared__ double As[2][BK][BM + 1];
+    __shared__ double Bs[2][BK][BN + 1];
+    __shared__ double sk[256], srow[BM], scol[BN];   // diagonal scalings, evaluated once per tile
+    typedef Mfma<double>::acc_t acc_t;
+
+    const int gm = (g.M + BM - 1) / BM, gn = (g.Nc + BN - 1) / BN;
+    int b, t;
+    chain_tile(gm * gn, g.batch, b, t);
+    const int m0 = (t % gm) * BM, n0 = (t / gm) * BN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const double* __restrict__ A = g.A + (long long)b * g.sA;
+    const double* __restrict__ B = g.B + (long long)b * g.sB;
+    double* __restrict__ C = g.C + (long long)b * g.sC;
+    const bool innd = g.innd.mode != DIAG_NONE;
+    if (innd)
+        for (int kk = tid; kk < g.K && kk < 256; kk += 256) sk[kk] = diag_value(g.innd, b, kk);
+    if (tid < BM) srow[tid] = (g.rowd.mode != DIAG_NONE && m0 + tid < g.M) ? diag_value(g.rowd, b, m0 + tid) : 1.0;
+    else
+        scol[tid - BM] = (g.cold.mode != DIAG_NONE && n0 + tid - BM < g.Nc) ? diag_value(g.cold, b, n0 + tid - BM) : 1.0;
+    __syncthreads();
+
+    acc_t acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    auto fetch = [&](int k0, double (&ra)[NA], double (&rb)[NB]) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = u * 256 + tid;
+            const int i = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
+            const int gi = m0 + i, gk = k0 + kk;
+            ra[u] = (gi < g.M && gk < g.K) ? (TA ? A[gk + (long long)gi * g.lda] : A[gi + (long long)gk * g.lda]) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = u * 256 + tid;
+            const int j = TB ? e % BN : e / BK, kk = TB ? e / BN : e % BK;
+            const int gj = n0 + j, gk = k0 + kk;
+            rb[u] = (gj < g.Nc && gk < g.K) ? (TB ? B[gj + (long long)gk * g.ldb] : B[gk + (long long)gj * g.ldb]) : 0.0;
+        }
+    };
+    auto stash = [&](int buf, int k0, const double (&ra)[NA], const double (&rb)[NB]) {
+#pragma unroll
+        for (int u = 0; u < NA; ++u) {
+            const int e = u * 256 + tid;
+            const int kk = TA ? e % BK : e / BM;
+            double v = ra[u];
+            if (innd && k0 + kk < g.K) v *= sk[k0 + kk];
+            As[buf][kk][TA ? e / BK : e % BM] = v;
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int e = u * 256 + tid;
+            Bs[buf][TB ? e / BN : e % BK][TB ? e % BN : e / BK] = rb[u];
+        }
+    };
+    auto compute = [&](int buf) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            double af[4], bf[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) af[i] = As[buf][kk + (lane >> 4)][wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bf[j] = Bs[buf][kk + (lane >> 4)][wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = Mfma<double>::mma(af[i], bf[j], acc[i][j]);
+        }
+    };
+
+    const int nk = (g.K + BK - 1) / BK;
+    double ra[NA], rb[NB];
+    fetch(0, ra, rb);
+    stash(0, 0, ra, rb);
+    __syncthreads();
+    // one register stage: step ks + 1 is loaded while step ks is multiplied from LDS buffer ks & 1
+    for (int ks = 0; ks < nk; ++ks) {
+        if (ks + 1 < nk) fetch((ks + 1) * BK, ra, rb);
+        compute(ks & 1);
+        if (ks + 1 < nk) stash((ks + 1) & 1, (ks + 1) * BK, ra, rb);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int gi = m0 + wm * WM + i * 16 + Mfma<double>::row(lane, r);
+                const int gj = n0 + wn * WN + j * 16 + (lane & 15);
+                if (gi < g.M && gj < g.Nc) {
+                    double v = acc[i][j][r] * g.alpha;
+                    if (g.rowd.mode != DIAG_NONE) v *= srow[gi - m0];
+                    if (g.cold.mode != DIAG_NONE) v *= scol[gj - n0];
+                    const long long off = g.tC ? (gj + (long long)gi * g.ldc) : (gi + (long long)gj * g.ldc);
+                    if (g.beta != 0.0) v += g.beta * C[off];
+                    C[off] = v;
+                }
+            }
+}
+
+template <bool TA, bool TB>
+static hipError_t launch_gemm128(const GemmArgs<double>& a, hipStream_t s)
+{
+    const long long blocks = (long long)((a.M + 127) / 128) * ((a.Nc + 127) / 128) * a.batch;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm128_kernel<TA, TB>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 template <typename T, bool TA, bool TB>
 static hipError_t launch_gemm64(const GemmArgs<T>& a, hipStream_t s)
 {
@@ -261,6 +386,17 @@ template <typename T>
 hipError_t launch_gemm(const GemmArgs<T>& a, int tile, hipStream_t s)
 {
     if (a.batch <= 0 || a.M <= 0 || a.Nc <= 0) return hipSuccess;
+    if (tile == TILE_128x128) {
+        // FP64 products with both output sides >= 128, K >= 64 and an inner scaling table that fits;
+        // other shapes (and FP32) take the 64 x 64 prefetching tile
+        if constexpr (std::is_same<T, double>::value) {
+            if (a.M >= 128 && a.Nc >= 128 && a.K >= 64 && (a.innd.mode == DIAG_NONE || a.K <= 256)) {
+                if (a.tA) return a.tB ? launch_gemm128<true, true>(a, s) : launch_gemm128<true, false>(a, s);
+                return a.tB ? launch_gemm128<false, true>(a, s) : launch_gemm128<false, false>(a, s);
+            }
+        }
+        tile = TILE_64x64_PF;
+    }
     switch (tile) {
         case TILE_32x32: return launch_gemm_tile<T, 32, 32>(a, s);
         case TILE_64x32: return launch_gemm_tile<T, 64, 32>(a, s);

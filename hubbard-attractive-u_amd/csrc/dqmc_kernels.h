@@ -77,7 +77,10 @@ hipError_t launch_kron_right(const KronArgs<T>& a, hipStream_t s);
 
 // Tile shapes the autotuner chooses from.
 // TILE_64x64_PF: 64 x 64 with double-buffered LDS and global loads two K steps ahead.
-enum GemmTile : int { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_64x64_PF = 3, TILE_COUNT = 4 };
+// TILE_128x128: FP64 128 x 128 (64 x 64 per wave) for products with both output sides >= 128 and
+// K >= 64, TILE_64x64_PF otherwise.
+enum GemmTile : int { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_64x64_PF = 3, TILE_128x128 = 4,
+                      TILE_COUNT = 5 };
 
 template <typename T>
 hipError_t launch_gemm(const GemmArgs<T>& a, int tile, hipStream_t s);

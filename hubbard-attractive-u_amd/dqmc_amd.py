@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdqmc_amd.so")
 
 FP64, MIXED = 0, 1
+TILE_NAMES = ["64x64", "32x32", "64x32", "64x64pf", "128x128"]   # GEMM tiles of dqmc_get/set_tuning
 GF_CURRENT, GF_TT, GF_T0, GF_0T = 0, 1, 2, 3
 HEALTH_NONFINITE_G, HEALTH_ZERO_PIVOT, HEALTH_RATIO, HEALTH_WATCHDOG = 1, 2, 4, 8
 OBS_FILLING, OBS_DOUBLE_OCCU, OBS_SWAVE, OBS_GREEN_KT, OBS_LDOS, OBS_SPIN = 1, 2, 4, 8, 16, 32
@@ -297,11 +298,11 @@ class Engine:
     def tuning(self):
         """Autotuned GEMM tiles {'fp64': name, 'fp32': name, 'ms': {(prec, name): ms}}."""
         a, b = C.c_int(), C.c_int()
-        ms = (C.c_float * 8)()
+        ms = (C.c_float * (2 * len(TILE_NAMES)))()
         self._chk(lib().dqmc_get_tuning(self._h, C.byref(a), C.byref(b), ms), "get_tuning")
-        names = ["64x64", "32x32", "64x32", "64x64pf"]
-        return {"fp64": names[a.value], "fp32": names[b.value],
-                "ms": {(p, names[t]): ms[i * 4 + t] for i, p in enumerate(("fp64", "fp32")) for t in range(4)}}
+        nt = len(TILE_NAMES)
+        return {"fp64": TILE_NAMES[a.value], "fp32": TILE_NAMES[b.value],
+                "ms": {(p, TILE_NAMES[t]): ms[i * nt + t] for i, p in enumerate(("fp64", "fp32")) for t in range(nt)}}
 
     def forward_fixed(self):
         """(logdet [chains], gdiag [chains, nt, ns]) at the current fields, no updates."""
@@ -326,8 +327,7 @@ class Engine:
         return out
 
     def set_tuning(self, fp64, fp32):
-        names = ["64x64", "32x32", "64x32", "64x64pf"]
-        self._chk(lib().dqmc_set_tuning(self._h, names.index(fp64), names.index(fp32)), "set_tuning")
+        self._chk(lib().dqmc_set_tuning(self._h, TILE_NAMES.index(fp64), TILE_NAMES.index(fp32)), "set_tuning")
 
     def event_elapsed(self, a, b):
         ms = C.c_float()

@@ -164,8 +164,9 @@ int dqmc_use_graphs(dqmc_ctx* ctx, int on);
 
 /* GEMM tile chosen by the autotuner in dqmc_init for FP64 (and FP32) products:
  * 0 = 64x64, 1 = 32x32, 2 = 64x32 output tiles, 3 = 64x64 with double-buffered LDS and global
- * loads two steps ahead; ms (optional, 2 x 4 floats) receives the measured time per wrap-shaped
- * product of every candidate.  No counterpart in the reference. */
+ * loads two steps ahead, 4 = 128x128 for FP64 products with both output sides >= 128 and an inner
+ * dimension >= 64 (tile 3 otherwise); ms (optional, 2 x 5 floats) receives the measured time per wrap-shaped product of
+ * every candidate.  No counterpart in the reference. */
 int dqmc_get_tuning(dqmc_ctx* ctx, int* tile_fp64, int* tile_fp32, float* ms);
 /* Override the autotuner's choice (after dqmc_init, before the first captured sweep). */
 int dqmc_set_tuning(dqmc_ctx* ctx, int tile_fp64, int tile_fp32);

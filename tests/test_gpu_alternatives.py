@@ -8,8 +8,9 @@ evaluation orders).  Workload: 16x16 (N = 256), U = -4, mu = -0.2, beta = 4, L =
 k = 32, 8 chains (a multiple of 8, so the chain-major block order of the batched kernels is the
 one exercised).
 
-  DQMC_GEMM_TILE=1|3     every GEMM with the 32x32 tile, or the 64x64 tile with double-buffered
-                         LDS and two-step global prefetch, instead of the autotuner's choice
+  DQMC_GEMM_TILE=1|3|4   every GEMM with the 32x32 tile, the 64x64 tile with double-buffered
+                         LDS and two-step global prefetch, or the FP64 128x128 tile, instead of
+                         the autotuner's choice
   DQMC_SOLVE=qr          recompose solve by Householder QR instead of partial-pivot LU
   DQMC_QR_PANEL=1024     QR panel by 16 waves (four threads per row) instead of 4 waves
   DQMC_QR_UPDATE=gemm    QR trailing update by three GEMM launches instead of one fused kernel
@@ -59,7 +60,7 @@ def default_run():
     return run_chains(2)
 
 
-@pytest.mark.parametrize("var,val", [("DQMC_GEMM_TILE", "1"), ("DQMC_GEMM_TILE", "3"), ("DQMC_SOLVE", "qr"), ("DQMC_QR_PANEL", "1024"), ("DQMC_QR_UPDATE", "gemm"), ("DQMC_WRAP", "dense"), ("DQMC_TRSM", "v1"),
+@pytest.mark.parametrize("var,val", [("DQMC_GEMM_TILE", "1"), ("DQMC_GEMM_TILE", "3"), ("DQMC_GEMM_TILE", "4"), ("DQMC_SOLVE", "qr"), ("DQMC_QR_PANEL", "1024"), ("DQMC_QR_UPDATE", "gemm"), ("DQMC_WRAP", "dense"), ("DQMC_TRSM", "v1"),
                                      ("DQMC_QR", "v1"), ("DQMC_METRO_THREADS", "256"),
                                      ("DQMC_METRO_THREADS", "512"), ("DQMC_LOOKAHEAD", "0")])
 def test_switch_follows_default_chains(default_run, monkeypatch, var, val):
