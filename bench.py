@@ -11,6 +11,7 @@ Metric: aggregate sweeps/s = B * steps / wall seconds of the timed region (input
 Also measured in the same run:
   * the whole sweep against the FP64 peak with the canonical algorithmic flop count,
   * the drop-in application's sweeps/s at 8x8, beta=8, L=80 with all six observables,
+  * the time of one fixed-field forward and one parameter adjoint over 256 configurations at 8x8, L=40,
   * the plain FP64 GPU form (1 chain, rank-1 Sherman-Morrison updates),
   * max |G - G_FP64| over the run (FP64 recompute at every stabilization),
   * the mixed FP32 form's |dG| on a short run (reported, not the value),
@@ -125,6 +126,23 @@ def mixed_table(gpu, paces, chains, sweeps):
     return "\n".join(rows)
 
 
+def mixed_batched_12x12(gpu, sweeps=2):
+    """Mixed precision on the batched workload: 64 chains of 12x12, U=-6, beta=6, L=120 (dt=0.05),
+    pace 10, mu in {-1, -0.5, 0} round-robin over the chains, FP32 wraps and flushes, k=32: the
+    largest max|G_mixed - G_FP64| over the chains and stabilizations of a short run."""
+    chains = 64
+    e = gpu.Engine(12, 120, 6.0, 6.0, mu=0.0, pace=10, n_chains=chains, precision=gpu.MIXED, delay_k=32)
+    for c in range(chains):
+        e.set_chain_mu(c, (-1.0, -0.5, 0.0)[c % 3])
+        e.set_seed(c, BASE_SEED + c)
+        e.fields_random(c)
+    e.init()
+    run_sweeps(e, sweeps, True)
+    dG = max(e.stats(c)["max_abs_dG"] for c in range(chains))
+    e.close()
+    return dG
+
+
 def app_throughput():
     """The drop-in application at the 8x8, U=-4, mu=-0.5, beta=8, L=80 (pace 10) configuration with
     all six observables measured on the device: directional sweeps per second of its simulation
@@ -175,6 +193,34 @@ def app_throughput():
             "workload": "app: 8x8 U=-4 mu=-0.5 beta=8 L=80 pace 10, 1 chain, all six observables on the device"}
 
 
+def adjoint_c5(gpu):
+    """The parameter adjoint on its workload (8x8, U=-4, mu=-0.3, beta=4, L=40, 256 field
+    configurations from seed): one fixed-field forward (log|det|, G(t)_ii of every configuration)
+    and one adjoint (the cotangents contracted with d/d(nnt, mu, u)), each timed after a warm-up call.
+    The adjoint runs L stabilized time-displaced passes per gradient."""
+    import numpy as np
+    nl, nt, beta, chains = 8, 40, 4.0, 256
+    e = gpu.Engine(nl, nt, beta, 4.0, mu=-0.3, nnt=1.0, pace=10, n_chains=chains, precision=gpu.FP64, delay_k=1)
+    for c in range(chains):
+        e.set_seed(c, BASE_SEED + c)
+        e.fields_random(c)
+    e.init()
+    gl = np.ones(chains)
+    gg = np.full((chains, nt, nl * nl), 1.0 / (nt * nl * nl))
+    e.forward_fixed()
+    e.adjoint(gl, gg)
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.forward_fixed()
+    t1 = time.perf_counter()
+    e.adjoint(gl, gg)
+    t2 = time.perf_counter()
+    e.close()
+    return {"workload": "adjoint: 8x8 U=-4 mu=-0.3 beta=4 L=40, 256 fixed configurations, FP64",
+            "forward_ms": round((t1 - t0) * 1e3, 3), "adjoint_ms": round((t2 - t1) * 1e3, 3),
+            "gradients_per_s": round(chains / (t2 - t0), 2)}
+
+
 # ----------------------------------------------------------------------------- GPU legs
 def make_engine(gpu, chains, precision, k):
     e = gpu.Engine(NL, NT, BETA, U, mu=MU, pace=PACE, n_chains=chains, precision=precision, delay_k=k)
@@ -217,6 +263,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 (1 chain, k=1) leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision check leg")
     ap.add_argument("--no-app", action="store_true", help="skip the drop-in application leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the parameter-adjoint timing leg")
     ap.add_argument("--mixed-table", action="store_true",
                     help="only print the table of mixed-precision |dG| against the stabilization pace")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -230,6 +277,8 @@ def main():
     if args.mixed_table:
         import dqmc_amd as gpu
         print(mixed_table(gpu, (1, 2, 4, 8), 8, 2))
+        print("\nbatched 12x12 (64 chains, U=-6, beta=6, L=120, pace 10, mu in {-1,-0.5,0}): max|G_mixed - G_FP64| "
+              "= %.3e" % mixed_batched_12x12(gpu))
         return
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
@@ -320,6 +369,11 @@ def main():
         mixed_dG = max(m.stats(c)["max_abs_dG"] for c in range(8))
         m.close()
 
+    adj = None
+    if not args.no_adjoint:
+        log("[bench] adjoint, 8x8 L=40, 256 configurations")
+        adj = adjoint_c5(gpu)
+
     app = None
     if not args.no_app:
         log("[bench] drop-in app, 8x8 beta=8 L=80, all observables")
@@ -351,6 +405,7 @@ def main():
         "roofline": roof,
         "sweep_roofline": sweep_roof,
         "app_c2": app,
+        "adjoint_c5": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
