@@ -107,7 +107,8 @@ struct dqmc_ctx {
     std::vector<void*> allocs;
     double *expK = nullptr, *invExpK = nullptr;
     float *expK32 = nullptr, *invExpK32 = nullptr;
-    double *E1 = nullptr, *F1 = nullptr;       // 16x16 padded exp(+-dt nnt A1) of the ring
+    double *E1 = nullptr, *F1 = nullptr;       // exp(+-dt nnt A1) of the ring, 16x16 (nl <= 16) or 32x32 padded
+    double *kS1 = nullptr, *kS2 = nullptr;     // transposes for the right product above 16 x 16
     float *E1f = nullptr, *F1f = nullptr;
     double *tab = nullptr, *itab = nullptr, *delta = nullptr;
     uint64_t *powtab = nullptr, *rng = nullptr;
@@ -311,6 +312,16 @@ int kron(dqmc_ctx* c, bool left, const T* X, T* C, const T* E, DiagArg rowd = Di
     k.nl = c->nl; k.N = c->N;
     k.rowd = rowd; k.innd = innd; k.cold = cold;
     k.batch = c->B;
+    if (c->nl > 16) {
+        // 17..32 sites per side: FP64 only (the engine keeps dense products for MIXED there)
+        if constexpr (std::is_same<T, double>::value) {
+            if (left) TIMED(c, KT_KRON, launch_kron32_left(k, c->stream));
+            else TIMED(c, KT_KRON, launch_kron32_right(k, c->kS1, c->kS2, c->stream));
+            return 0;
+        } else {
+            return c->fail(DQMC_ESTATE, "kron: FP32 products need nl <= 16");
+        }
+    }
     if (left) TIMED(c, KT_KRON, launch_kron_left<T>(k, c->stream));
     else TIMED(c, KT_KRON, launch_kron_right<T>(k, c->stream));
     return 0;
@@ -870,9 +881,10 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         const char* w = std::getenv("DQMC_WRAP");
         c->kron = !(w && std::string(w) == "dense");
         if (N > 256) {
-            // Lattices above 16 x 16: dense slice products (the ring factor is 16 x 16), the
-            // Householder QR solve with the wide panel kernel and the three-GEMM update
-            c->kron = false;
+            // Lattices above 16 x 16: factored slice products on the 32 x 32 padded ring factor in
+            // FP64 (dense GEMMs for MIXED), the Householder QR solve with the wide panel kernel and
+            // the three-GEMM update
+            if (precision == DQMC_MIXED) c->kron = false;
             c->solve_lu = false;
             c->qr_update = QR_UPDATE_GEMM;
             c->lookahead = false;
@@ -895,7 +907,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     HCHECK(c, prepare_kernels());
     for (auto& e : c->ev) HCHECK(c, hipEventCreate(&e));
     RCHECK(c->alloc(&c->expK, nn * 8)); RCHECK(c->alloc(&c->invExpK, nn * 8));
-    RCHECK(c->alloc(&c->E1, 256 * 8)); RCHECK(c->alloc(&c->F1, 256 * 8));
+    RCHECK(c->alloc(&c->E1, 1024 * 8)); RCHECK(c->alloc(&c->F1, 1024 * 8));
+    if (c->kron && N > 256) { RCHECK(c->alloc(&c->kS1, (size_t)B * nn * 8)); RCHECK(c->alloc(&c->kS2, (size_t)B * nn * 8)); }
     RCHECK(c->alloc(&c->E1f, 256 * 4)); RCHECK(c->alloc(&c->F1f, 256 * 4));
     RCHECK(c->alloc(&c->tab, (size_t)B * 16)); RCHECK(c->alloc(&c->itab, (size_t)B * 16));
     RCHECK(c->alloc(&c->delta, (size_t)B * 16));
@@ -948,13 +961,15 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         HCHECK(c, hipMemcpy(c->invExpK32, f.data(), nn * 4, hipMemcpyHostToDevice));
     }
     {
-        std::vector<double> e1(256), f1(256);
-        ring_expk16(c->nl, c->dt * c->nnt, e1.data());
-        ring_expk16(c->nl, -c->dt * c->nnt, f1.data());
+        // 16 x 16 padded up to 16 sites per side, 32 x 32 above (the FP32 copies serve nl <= 16 only)
+        const int P = c->nl > 16 ? 32 : 16;
+        std::vector<double> e1(1024), f1(1024);
+        ring_expk_padded(c->nl, c->dt * c->nnt, P, e1.data());
+        ring_expk_padded(c->nl, -c->dt * c->nnt, P, f1.data());
         std::vector<float> e1f(256), f1f(256);
         for (int i = 0; i < 256; ++i) { e1f[i] = (float)e1[i]; f1f[i] = (float)f1[i]; }
-        HCHECK(c, hipMemcpy(c->E1, e1.data(), 256 * 8, hipMemcpyHostToDevice));
-        HCHECK(c, hipMemcpy(c->F1, f1.data(), 256 * 8, hipMemcpyHostToDevice));
+        HCHECK(c, hipMemcpy(c->E1, e1.data(), 1024 * 8, hipMemcpyHostToDevice));
+        HCHECK(c, hipMemcpy(c->F1, f1.data(), 1024 * 8, hipMemcpyHostToDevice));
         HCHECK(c, hipMemcpy(c->E1f, e1f.data(), 256 * 4, hipMemcpyHostToDevice));
         HCHECK(c, hipMemcpy(c->F1f, f1f.data(), 256 * 4, hipMemcpyHostToDevice));
     }

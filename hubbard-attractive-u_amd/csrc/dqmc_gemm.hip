@@ -728,12 +728,12 @@ hipError_t launch_kron_right(const KronArgs<T>& a, hipStream_t s)
 // 2 and nl = 1 gives 2 on the diagonal), padded to a 16 x 16 row-major block.  The square
 // lattice's exp(x A) is its Kronecker square.  Host code; rings of more than 16 sites have no
 // padded factor (their products are dense GEMMs) and leave out zero.
-void ring_expk16(int nl, double x, double* out)
+void ring_expk_padded(int nl, double x, int P, double* out)
 {
-    for (int e = 0; e < 256; ++e) out[e] = 0.0;
-    if (nl > 16) return;
+    for (int e = 0; e < P * P; ++e) out[e] = 0.0;
+    if (nl > P) return;
     if (x == 0.0) {   // exactly the identity (the cosine sums below round to ~1e-17)
-        for (int i = 0; i < nl; ++i) out[i * 16 + i] = 1.0;
+        for (int i = 0; i < nl; ++i) out[i * P + i] = 1.0;
         return;
     }
     for (int i = 0; i < nl; ++i)
@@ -743,8 +743,133 @@ void ring_expk16(int nl, double x, double* out)
                 const double k = 2.0 * M_PI * m / nl;
                 s += std::cos(k * (i - j)) * std::exp(2.0 * x * std::cos(k));
             }
-            out[i * 16 + j] = s / nl;
+            out[i * P + j] = s / nl;
         }
+}
+void ring_expk16(int nl, double x, double* out) { ring_expk_padded(nl, x, 16, out); }
+
+// Left product on lattices of 17..32 sites per side.  With E and every lattice vector V padded to
+// 32 x 32 and split into 16 x 16 blocks, (E(x)E) v = E V E^T is computed as P = E V (P^T blocks
+// accumulated from V^T E^T) and Z = P E^T, 32 + 32 MFMAs (16 x 16 x 4) per vector; as in the 16 x 16
+// kernels, each block of P^T is already the A-fragment of the second step.  One wave per column of X:
+// the column is read coalesced into the wave's LDS tile t[y][x] (row stride 33), the fragments are read
+// from there, and Z goes back through the same tile for a coalesced store.  The diagonal scalings are
+// applied at the load (inner) and the store (rows, column).
+__global__ __launch_bounds__(256) void kron_left32_kernel(KronArgs<double> a)
+{
+    __shared__ double tl[4][32 * 33];
+    typedef Mfma<double>::acc_t acc_t;
+    const int b = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nl = a.nl, N = a.N, l16 = lane & 15, lq = lane >> 4;
+    const int c = blockIdx.x * 4 + wave;
+    if (c >= N) return;   // whole waves; the kernel has no workgroup barrier
+    double* t = tl[wave];
+    // B-fragments of E: e[p][q][s] holds E(16 p + l16, 16 q + lq + 4 s), i.e. B[k][col] of block (p, q)
+    double e[2][2][4];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) e[p][q][s] = a.E[(16 * p + l16) * 32 + 16 * q + lq + 4 * s];
+    for (int i = lane; i < 32 * 33; i += 64) t[i] = 0.0;
+    wave_lds_sync();
+    const double* X = a.X + (long long)b * a.sX + (long long)c * N;
+    for (int site = lane; site < N; site += 64) {
+        double v = X[site];
+        if (a.innd.mode != DIAG_NONE) v *= diag_value(a.innd, b, site);
+        t[(site / nl) * 33 + site % nl] = v;
+    }
+    wave_lds_sync();
+    // step 1: acc1[bp][p] = block (y' in bp, x in p) of P^T; A(y', x') = V(x', y') = t[y'][x']
+    acc_t acc1[2][2];
+#pragma unroll
+    for (int bp = 0; bp < 2; ++bp)
+#pragma unroll
+        for (int p = 0; p < 2; ++p) acc1[bp][p] = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int bp = 0; bp < 2; ++bp)
+#pragma unroll
+        for (int cp = 0; cp < 2; ++cp)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const double vf = t[(16 * bp + l16) * 33 + 16 * cp + lq + 4 * s];
+#pragma unroll
+                for (int p = 0; p < 2; ++p) acc1[bp][p] = Mfma<double>::mma(vf, e[p][cp][s], acc1[bp][p]);
+            }
+    // step 2: acc2[p][q] = block (x in p, y in q) of Z; item s of acc1[bp][p] is A(x, y' = 16 bp + 4 s + lq)
+    acc_t acc2[2][2];
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            acc_t z = acc_t{0, 0, 0, 0};
+#pragma unroll
+            for (int bp = 0; bp < 2; ++bp)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) z = Mfma<double>::mma(acc1[bp][p][s], e[q][bp][s], z);
+            acc2[p][q] = z;
+        }
+    wave_lds_sync();   // every fragment read precedes the writes of Z into the tile
+#pragma unroll
+    for (int p = 0; p < 2; ++p)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) t[(16 * q + l16) * 33 + 16 * p + 4 * r + lq] = acc2[p][q][r];
+    wave_lds_sync();
+    double* C = a.C + (long long)b * a.sC + (long long)c * N;
+    const double cs = (a.cold.mode != DIAG_NONE) ? diag_value(a.cold, b, c) : 1.0;
+    for (int site = lane; site < N; site += 64) {
+        double v = t[(site / nl) * 33 + site % nl] * cs;
+        if (a.rowd.mode != DIAG_NONE) v *= diag_value(a.rowd, b, site);
+        C[site] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void transpose_kernel(const double* __restrict__ Xs, long long sX,
+                                                        double* __restrict__ Ys, long long sY, int N)
+{
+    __shared__ double tt[32][33];
+    const int b = blockIdx.y, nt = (N + 31) / 32;
+    const int i0 = (blockIdx.x % nt) * 32, j0 = (blockIdx.x / nt) * 32;
+    const double* X = Xs + (long long)b * sX;
+    double* Y = Ys + (long long)b * sY;
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        const int ii = e & 31, jj = e >> 5;   // X(i0 + ii, j0 + jj): consecutive ii contiguous
+        tt[jj][ii] = (i0 + ii < N && j0 + jj < N) ? X[(i0 + ii) + (long long)(j0 + jj) * N] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        const int jj = e & 31, ii = e >> 5;   // Y(j0 + jj, i0 + ii) = X(i0 + ii, j0 + jj)
+        if (i0 + ii < N && j0 + jj < N) Y[(j0 + jj) + (long long)(i0 + ii) * N] = tt[jj][ii];
+    }
+}
+
+hipError_t launch_transpose(const double* X, long long sX, double* Y, long long sY, int N, int batch, hipStream_t s)
+{
+    const int nt = (N + 31) / 32;
+    hipLaunchKernelGGL(transpose_kernel, dim3(nt * nt, batch), dim3(256), 0, s, X, sX, Y, sY, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_kron32_left(const KronArgs<double>& a, hipStream_t s)
+{
+    if (a.nl < 1 || a.nl > 32 || a.N != a.nl * a.nl) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kron_left32_kernel, dim3((a.N + 3) / 4, a.batch), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_kron32_right(const KronArgs<double>& a, double* S1, double* S2, hipStream_t s)
+{
+    hipError_t e = launch_transpose(a.X, a.sX, S1, a.sX, a.N, a.batch, s);
+    if (e != hipSuccess) return e;
+    KronArgs<double> t = a;
+    t.X = S1; t.C = S2; t.sC = a.sX;
+    t.rowd = a.cold;   // C^T = diag(cv) (E(x)E) diag(s) X^T diag(r)
+    t.cold = a.rowd;
+    if ((e = launch_kron32_left(t, s)) != hipSuccess) return e;
+    return launch_transpose(S2, a.sX, a.C, a.sC, a.N, a.batch, s);
 }
 
 template hipError_t launch_kron_left<double>(const KronArgs<double>&, hipStream_t);

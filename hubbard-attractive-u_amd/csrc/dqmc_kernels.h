@@ -68,8 +68,18 @@ struct KronArgs {
 int free_error(int code, const char* msg);
 const char* free_error_text();
 
-// Host: the 16 x 16 padded factor E = exp(x A1) of the ring of nl sites (x = 0: the identity).
+// Host: the factor E = exp(x A1) of the ring of nl sites (x = 0: the identity), padded with zeros to a
+// P x P row-major block (P * P entries; nl <= P); ring_expk16 is P = 16 and leaves zeros for nl > 16.
+void ring_expk_padded(int nl, double x, int P, double* out);
 void ring_expk16(int nl, double x, double* out);
+// Lattices of 17..32 sites per side (FP64), E padded to 32 x 32: the left product on 32 x 32 tiles of
+// 2 x 2 MFMA blocks; the right product applies (E(x)E) to the rows through two batched transposes
+// into S1 and S2 (N x N per chain at the chain stride of X): C^T = Left(X^T) with the row and column
+// scalings exchanged.
+hipError_t launch_kron32_left(const KronArgs<double>& a, hipStream_t s);
+hipError_t launch_kron32_right(const KronArgs<double>& a, double* S1, double* S2, hipStream_t s);
+// Y_b = X_b^T for N x N column-major matrices, through 32 x 32 LDS tiles.
+hipError_t launch_transpose(const double* X, long long sX, double* Y, long long sY, int N, int batch, hipStream_t s);
 template <typename T>
 hipError_t launch_kron_left(const KronArgs<T>& a, hipStream_t s);
 template <typename T>

@@ -65,7 +65,7 @@ enum {
  * Sherman-Morrison update (src/hubbard.cpp:132-139).  The accept/reject sequence does not
  * depend on delay_k.  Lattices: 1 <= nl <= 32 (N <= 1024).  Above 16 x 16 the pending update
  * of a block is limited by LDS, so delay_k is capped (FP64: 24 at nl = 18, 12 at nl = 24, 8 at
- * nl = 32) and the slice products are dense MFMA GEMMs.
+ * nl = 32); the FP64 slice products there use the ring factor padded to 32 x 32.
  * Replaces: construction of DQMC::Core + Model::HubbardAttractiveU + SquareLattice and their
  * initialize() calls (src/main.cpp:119-128; src/dqmc_core.cpp:20-30; src/hubbard.cpp:25-48). */
 int dqmc_create(const dqmc_params_c* params, int n_chains, int precision, int delay_k, dqmc_ctx** out);

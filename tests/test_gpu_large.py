@@ -1,7 +1,8 @@
 """GPU parity on lattices above 16 x 16 (N up to 1024) and with is_fft, through the C ABI, against the
 FP64 CPU oracle on the same seeds and fields.
 
-Above 16 x 16 the engine runs dense MFMA slice products, the wide QR panel kernel (8 waves, up to two
+Above 16 x 16 the engine runs the slice products factored on a 32 x 32 zero-padded ring factor (the
+right product through two batched transposes), the wide QR panel kernel (8 waves, up to two
 rows per thread) with the three-GEMM trailing update, the Householder QR recompose solve and a delay
 capped by LDS (src/hubbard.cpp:25-48 and :293-341; src/dqmc_core.cpp:167-284; the reference accepts any
 nl >= 1, include/dqmc_parser.hpp:125).  With is_fft the reference forms exp(-+dt K) through the lattice
@@ -63,6 +64,13 @@ def test_large_lattice_oracle_parity(nl, nt, beta, pace, k, nch):
     d = float(np.abs(e.gf(pick) - o.gf(0)).max())
     say("  init max|dG| = %.3e" % d)
     assert d < 1e-9
+    # the five slice operators B, B^-1, left and right, and B^T (src/hubbard.cpp:293-341)
+    g = np.random.default_rng(nl).standard_normal((nl * nl, nl * nl))
+    for op in range(5):
+        want = o.apply_bop(op, nt - 2, g)
+        err = float(np.abs(e.apply_bop(pick, op, nt - 2, g) - want).max() / np.abs(want).max())
+        say("  B-op %d: rel err %.2e" % (op, err))
+        assert err < 1e-13
     for sweep in range(2):
         up = sweep % 2 == 0
         e.sweep_up() if up else e.sweep_down()
@@ -95,7 +103,7 @@ def test_largest_lattice_32x32():
     e.close()
 
 
-@pytest.mark.parametrize("nl,k", [(6, 8), (16, 32)])
+@pytest.mark.parametrize("nl,k", [(6, 8), (16, 32), (20, 16)])
 def test_is_fft_matches_oracle_fft_products(nl, k):
     nt, beta, u, mu, pace, seed = 20, 2.0, 4.0, -0.4, 5, 515
     say("case: is_fft nl=%d k=%d" % (nl, k))
