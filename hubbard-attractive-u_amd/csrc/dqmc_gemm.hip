@@ -725,9 +725,8 @@ hipError_t launch_kron_right(const KronArgs<T>& a, hipStream_t s)
     return hipGetLastError();
 }
 // exp(x A1) for the periodic ring of nl sites (A1[i][i+1] += 1, A1[i+1][i] += 1, so nl = 2 gives
-// 2 and nl = 1 gives 2 on the diagonal), padded to a 16 x 16 row-major block.  The square
-// lattice's exp(x A) is its Kronecker square.  Host code; rings of more than 16 sites have no
-// padded factor (their products are dense GEMMs) and leave out zero.
+// 2 and nl = 1 gives 2 on the diagonal), padded with zeros to a P x P row-major block.  The square
+// lattice's exp(x A) is its Kronecker square.  Host code; a ring of more than P sites leaves out zero.
 void ring_expk_padded(int nl, double x, int P, double* out)
 {
     for (int e = 0; e < P * P; ++e) out[e] = 0.0;
