@@ -118,6 +118,9 @@ struct dqmc_ctx {
     double *G64 = nullptr;
     float *G32 = nullptr, *T32 = nullptr;
     double *stkA = nullptr, *stkB = nullptr, *stk = nullptr, *stk_alt = nullptr;
+    double* ident = nullptr;               // one N x N identity, read at batch stride 0
+    DStack* chain_stack = nullptr;         // the stack the current B-chain block is pushed onto
+    bool chain_first = false;              // no factor of the block applied yet
     double *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *qr = nullptr;
     int* perm = nullptr;
     double *gt0A = nullptr, *gt0B = nullptr, *g0tA = nullptr, *g0tB = nullptr, *gt0 = nullptr, *g0t = nullptr;
@@ -305,10 +308,10 @@ hipError_t factor_any(dqmc_ctx* c, int W)
 // C = diag(r) (E(x)E) diag(s) X diag(cv)  (left) or  diag(r) X diag(s) (E(x)E) diag(cv)  (right)
 template <typename T>
 int kron(dqmc_ctx* c, bool left, const T* X, T* C, const T* E, DiagArg rowd = DiagArg(), DiagArg innd = DiagArg(),
-         DiagArg cold = DiagArg())
+         DiagArg cold = DiagArg(), long long sX = -1)
 {
     KronArgs<T> k;
-    k.X = X; k.sX = c->NN(); k.C = C; k.sC = c->NN(); k.E = E;
+    k.X = X; k.sX = sX < 0 ? c->NN() : sX; k.C = C; k.sC = c->NN(); k.E = E;
     k.nl = c->nl; k.N = c->N;
     k.rowd = rowd; k.innd = innd; k.cold = cold;
     k.batch = c->B;
@@ -401,12 +404,6 @@ int right_expk(dqmc_ctx* c, const double* X, double* C, const double* table, int
                         inner ? d : dnone(), inner ? dnone() : d);
 }
 // stacks <- B(t) stacks (src/dqmc_core.cpp:190) or B(t)^T stacks (:251, adjB, src/hubbard.cpp:333-341)
-int chain_mul(dqmc_ctx* c, int t, bool adjoint)
-{
-    RCHECK(left_expk(c, c->stk, c->stk_alt, c->tab, t, adjoint));
-    std::swap(c->stk, c->stk_alt);
-    return 0;
-}
 int metropolis(dqmc_ctx* c, int t)
 {
     if (c->prec == DQMC_MIXED) {
@@ -432,18 +429,47 @@ double* sU(dqmc_ctx* c, DStack& s, int k) { return s.U + (long long)k * c->NN();
 double* sV(dqmc_ctx* c, DStack& s, int k) { return s.V + (long long)k * c->NN(); }
 double* sD(dqmc_ctx* c, DStack& s, int k) { return s.D + (long long)k * c->N; }
 
+// Start the B-chain of a stack block that will be pushed onto s.
+void chain_begin(dqmc_ctx* c, DStack& s)
+{
+    c->stk = c->stkA;
+    c->stk_alt = c->stkB;
+    c->chain_stack = &s;
+    c->chain_first = true;
+}
+// One factor of the block's B-chain: stk <- B(t) stk (src/dqmc_core.cpp:190) or B(t)^T stk (:251, adjB,
+// src/hubbard.cpp:333-341).  The first factor of a block reads the top U of the stack the block goes to
+// (the identity when that stack is empty) instead of an identity-initialized chain, and the last one
+// (`last`) scales the columns by that stack's top D, so the push receives (chain * U_top) * diag(D_top)
+// (svd_stack.hpp:144) with no separate N x N x N product.
+int chain_mul(dqmc_ctx* c, int t, bool adjoint, bool last)
+{
+    DStack& s = *c->chain_stack;
+    const long long nn = c->NN();
+    const double* X = c->stk;
+    long long sX = nn;
+    if (c->chain_first) {
+        if (s.len > 0) { X = sU(c, s, s.len - 1); sX = (long long)c->S * nn; }
+        else { X = c->ident; sX = 0; }
+    }
+    const DiagArg d = dfield(c, c->tab, t);
+    const DiagArg rowd = adjoint ? dnone() : d, innd = adjoint ? d : dnone();
+    const DiagArg cold = (last && s.len > 0) ? dvec(DIAG_VEC, sD(c, s, s.len - 1), (long long)c->S * c->N) : dnone();
+    if (c->kron) RCHECK(kron<double>(c, true, X, c->stk_alt, c->E1, rowd, innd, cold, sX));
+    else RCHECK(gemm<double>(c, c->expK, 0, c->N, 0, X, sX, c->N, 0, c->stk_alt, nn, c->N, 0, rowd, innd, cold));
+    std::swap(c->stk, c->stk_alt);
+    c->chain_first = false;
+    return 0;
+}
+
 int stack_push(dqmc_ctx* c, DStack& s)
 {
     const long long nn = c->NN(), st = (long long)c->S * nn, sd = (long long)c->S * c->N;
     const int N = c->N;
     if (s.len >= c->S) return c->fail(DQMC_ESTATE, "stack overflow");
+    if (c->chain_stack != &s || c->chain_first) return c->fail(DQMC_ESTATE, "stack push without its B-chain");
+    // M = (stacks * U_top) * diag(D_top) (svd_stack.hpp:144), formed by the block's chain_mul calls
     const double* M = c->stk;
-    if (s.len > 0) {
-        // M = (stacks * U_top) * diag(D_top)   (svd_stack.hpp:144)
-        RCHECK(gemm<double>(c, c->stk, nn, N, 0, sU(c, s, s.len - 1), st, N, 0, c->W1, nn, N, 0, dnone(), dnone(),
-                            dvec(DIAG_VEC, sD(c, s, s.len - 1), sd)));
-        M = c->W1;
-    }
     HCHECK(c, launch_udv_prepare(M, nn, c->qr, c->sqr(), c->weq(), c->perm, N, c->B, c->stream));
     TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, c->weq(), c->B, c->stream));
     HCHECK(c, launch_udv_finish(c->qr, c->sqr(), c->weq(), c->perm, sU(c, s, s.len), st, sD(c, s, s.len), sd, c->W2, nn, N,
@@ -537,14 +563,6 @@ int recompute_dyn(dqmc_ctx* c, double* gtt, double* gt0n, double* g0tn)
     return 0;
 }
 
-int set_identity_stk(dqmc_ctx* c)
-{
-    c->stk = c->stkA;
-    c->stk_alt = c->stkB;
-    HCHECK(c, launch_set_identity(c->stk, c->NN(), c->N, 1.0, c->B, c->stream));
-    return 0;
-}
-
 int store_current(dqmc_ctx* c, double* vec, int t)
 {
     const long long nn = c->NN(), sv = (long long)c->L * nn;
@@ -560,7 +578,7 @@ int stabilize_eq(dqmc_ctx* c)
     RCHECK(recompute_eq(c, c->W3));
     HCHECK(c, launch_compare_commit(c->W3, c->G64, c->prec == DQMC_MIXED ? c->G32 : nullptr, c->NN(), c->N, c->wrap_eq,
                                     c->maxdG, c->wd_bound, c->health, c->cmp_part, c->B, c->stream));
-    return set_identity_stk(c);
+    return 0;
 }
 
 // ---- sweeps ---------------------------------------------------------------------------------
@@ -568,15 +586,17 @@ int do_sweep_up(dqmc_ctx* c)
 {
     const int L = c->L;
     const bool store = !c->thermalization && c->eqtime;
-    RCHECK(set_identity_stk(c));
+    chain_begin(c, c->left);
     for (int t = 1; t <= L; ++t) {
+        const bool push = t % c->pace == 0 || t == L;
         RCHECK(wrap_up(c, t % L));
         RCHECK(metropolis(c, t % L));
-        RCHECK(chain_mul(c, t % L, false));
-        if (t % c->pace == 0 || t == L) {
+        RCHECK(chain_mul(c, t % L, false, push));
+        if (push) {
             c->right.len -= 1;
             RCHECK(stack_push(c, c->left));
             RCHECK(stabilize_eq(c));
+            chain_begin(c, c->left);
         }
         if (store) RCHECK(store_current(c, c->vecgftt, t % L));
     }
@@ -587,15 +607,17 @@ int do_sweep_down(dqmc_ctx* c)
 {
     const int L = c->L;
     const bool store = !c->thermalization && c->eqtime;
-    RCHECK(set_identity_stk(c));
+    chain_begin(c, c->right);
     for (int t = L; t >= 1; --t) {
+        const bool push = (t - 1) % c->pace == 0;
         RCHECK(metropolis(c, t % L));
-        RCHECK(chain_mul(c, t % L, true));
+        RCHECK(chain_mul(c, t % L, true, push));
         RCHECK(wrap_down(c, t % L));
-        if ((t - 1) % c->pace == 0) {
+        if (push) {
             c->left.len -= 1;
             RCHECK(stack_push(c, c->right));
             RCHECK(stabilize_eq(c));
+            chain_begin(c, c->right);
         }
         if (store) RCHECK(store_current(c, c->vecgftt, (t - 1) % L));
     }
@@ -611,17 +633,18 @@ int do_sweep_dynamic(dqmc_ctx* c)
     else HCHECK(c, launch_copy_d(c->G64, nn, c->W1, nn, nn, c->B, c->stream));
     HCHECK(c, launch_copy_d(c->W1, nn, c->gt0, nn, nn, c->B, c->stream));
     HCHECK(c, launch_axpi(c->W1, nn, c->g0t, nn, 1.0, -1.0, N, c->B, c->stream));
-    RCHECK(set_identity_stk(c));
+    chain_begin(c, c->left);
     for (int t = 1; t <= L; ++t) {
         const int ti = t % L;
+        const bool push = t % c->pace == 0 || t == L;
         RCHECK(wrap_up(c, ti));
         double* gt0n = (c->gt0 == c->gt0A) ? c->gt0B : c->gt0A;
         double* g0tn = (c->g0t == c->g0tA) ? c->g0tB : c->g0tA;
         RCHECK(left_expk(c, c->gt0, gt0n, c->tab, ti, false));
         RCHECK(right_expk(c, c->g0t, g0tn, c->itab, ti, false));
         c->gt0 = gt0n; c->g0t = g0tn;
-        RCHECK(chain_mul(c, ti, false));
-        if (t % c->pace == 0 || t == L) {
+        RCHECK(chain_mul(c, ti, false, push));
+        if (push) {
             c->right.len -= 1;
             RCHECK(stack_push(c, c->left));
             RCHECK(recompute_dyn(c, c->W3, c->W2, c->W1));
@@ -631,7 +654,7 @@ int do_sweep_dynamic(dqmc_ctx* c)
             HCHECK(c, launch_compare_only(c->W1, c->g0t, nn, N, c->wrap_dyn, c->health, c->B, c->stream));
             HCHECK(c, launch_copy_d(c->W2, nn, c->gt0, nn, nn, c->B, c->stream));
             HCHECK(c, launch_copy_d(c->W1, nn, c->g0t, nn, nn, c->B, c->stream));
-            RCHECK(set_identity_stk(c));
+            chain_begin(c, c->left);
         }
         RCHECK(store_current(c, c->vecgftt, ti));
         if (ti != 0) {
@@ -653,12 +676,13 @@ int build_from_fields(dqmc_ctx* c)
     const int L = c->L;
     const long long nn = c->NN();
     c->left.len = 0; c->right.len = 0; c->current_t = 0;
-    RCHECK(set_identity_stk(c));
+    chain_begin(c, c->right);
     for (int t = L; t >= 1; --t) {
-        RCHECK(chain_mul(c, t % L, true));
-        if ((t - 1) % c->pace == 0) {
+        const bool push = (t - 1) % c->pace == 0;
+        RCHECK(chain_mul(c, t % L, true, push));
+        if (push) {
             RCHECK(stack_push(c, c->right));
-            RCHECK(set_identity_stk(c));
+            chain_begin(c, c->right);
         }
     }
     RCHECK(recompute_eq(c, c->W3));
@@ -909,6 +933,7 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     RCHECK(c->alloc(&c->expK, nn * 8)); RCHECK(c->alloc(&c->invExpK, nn * 8));
     RCHECK(c->alloc(&c->E1, 1024 * 8)); RCHECK(c->alloc(&c->F1, 1024 * 8));
     if (c->kron && N > 256) { RCHECK(c->alloc(&c->kS1, (size_t)B * nn * 8)); RCHECK(c->alloc(&c->kS2, (size_t)B * nn * 8)); }
+    RCHECK(c->alloc(&c->ident, nn * 8));
     RCHECK(c->alloc(&c->E1f, 256 * 4)); RCHECK(c->alloc(&c->F1f, 256 * 4));
     RCHECK(c->alloc(&c->tab, (size_t)B * 16)); RCHECK(c->alloc(&c->itab, (size_t)B * 16));
     RCHECK(c->alloc(&c->delta, (size_t)B * 16));
@@ -953,6 +978,11 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     lattice_expk(c->nl, -c->dt * c->nnt, iek);  // exp(+dt K) at mu = 0
     HCHECK(c, hipMemcpy(c->expK, ek.data(), nn * 8, hipMemcpyHostToDevice));
     HCHECK(c, hipMemcpy(c->invExpK, iek.data(), nn * 8, hipMemcpyHostToDevice));
+    {
+        std::vector<double> id((size_t)nn, 0.0);
+        for (int i = 0; i < N; ++i) id[(size_t)i * N + i] = 1.0;
+        HCHECK(c, hipMemcpy(c->ident, id.data(), nn * 8, hipMemcpyHostToDevice));
+    }
     if (precision == DQMC_MIXED) {
         std::vector<float> f(nn);
         for (long long i = 0; i < nn; ++i) f[i] = (float)ek[i];
