@@ -1147,9 +1147,9 @@ hipError_t launch_lu_blocked(double* buf, long long stride, int N, int W, int ba
 
 // ---------------------------------------------------------------------------------------------
 //  Blocked back-substitution X = R^{-1} Z for the row-major system (R in columns [0, N), Z in
-//  columns [c0, c0+P), solved in place), 32-row blocks from the bottom:
+//  columns [c0, c0+P), solved in place), 32-row blocks, recursively from the bottom half:
 //    trsm_diag_kernel  X_k = R_kk^{-1} Z_k, one thread per right-hand side, R_kk in LDS;
-//    GEMM (MFMA)       Z_{0:k} -= R_{0:k, k} X_k, as Z^T_{:, 0:k} -= X_k^T R_{0:k, k}^T.
+//    GEMM (MFMA)       Z_lo -= R_{lo, hi} X_hi, as Z^T_{:, lo} -= X_hi^T R_{lo, hi}^T.
 // ---------------------------------------------------------------------------------------------
 #define TRB 32
 
@@ -1184,28 +1184,37 @@ __global__ __launch_bounds__(256) void trsm_diag_kernel(double* __restrict__ buf
         if (m < nbk) Z[(long long)(k0 + m) * W] = x[m];
 }
 
+// Blocks [lo, hi) of 32 rows: the lower half of the range is updated with the solved upper half in
+// one GEMM, Z_{lo:mid} -= R_{lo:mid, mid:hi} X_{mid:hi}, so each level of the recursion passes over half of
+// Z once (three passes at N = 256, against seven rank-32 updates of growing height bottom-up).
+static hipError_t trsm_range(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile, int lo,
+                             int hi, hipStream_t s)
+{
+    if (hi - lo == 1) {
+        const int k0 = lo * TRB, nbk = (N - k0 < TRB) ? N - k0 : TRB;
+        hipLaunchKernelGGL(trsm_diag_kernel, dim3((P + 255) / 256, batch), dim3(256), 0, s, buf, stride, W, c0, P, k0,
+                           nbk);
+        return hipGetLastError();
+    }
+    const int mid = (lo + hi) / 2;
+    hipError_t e = trsm_range(buf, stride, N, W, c0, P, batch, tile, mid, hi, s);
+    if (e != hipSuccess) return e;
+    const int r0 = lo * TRB, s0 = mid * TRB, s1 = (hi * TRB < N) ? hi * TRB : N;
+    GemmArgs<double> g;
+    g.M = P; g.Nc = s0 - r0; g.K = s1 - s0;
+    g.A = buf + c0 + (long long)s0 * W; g.sA = stride; g.lda = W; g.tA = 0;   // X_{mid:hi}^T
+    g.B = buf + (long long)r0 * W + s0; g.sB = stride; g.ldb = W; g.tB = 0;   // R_{lo:mid, mid:hi}^T
+    g.C = buf + c0 + (long long)r0 * W; g.sC = stride; g.ldc = W; g.tC = 0;   // Z_{lo:mid}^T
+    g.alpha = -1.0; g.beta = 1.0; g.batch = batch;
+    if ((e = launch_gemm<double>(g, tile, s)) != hipSuccess) return e;
+    return trsm_range(buf, stride, N, W, c0, P, batch, tile, lo, mid, s);
+}
+
 hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,
                                hipStream_t s)
 {
     if (N < 1 || P < 1 || c0 < N || c0 + P > W) return hipErrorInvalidValue;
-    const int nblk = (N + TRB - 1) / TRB;
-    for (int kb = nblk - 1; kb >= 0; --kb) {
-        const int k0 = kb * TRB, nbk = (N - k0 < TRB) ? N - k0 : TRB;
-        hipLaunchKernelGGL(trsm_diag_kernel, dim3((P + 255) / 256, batch), dim3(256), 0, s, buf, stride, W, c0, P, k0,
-                           nbk);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        if (k0 == 0) break;
-        GemmArgs<double> g;
-        g.M = P; g.Nc = k0; g.K = nbk;
-        g.A = buf + c0 + (long long)k0 * W; g.sA = stride; g.lda = W; g.tA = 0;
-        g.B = buf + k0; g.sB = stride; g.ldb = W; g.tB = 0;
-        g.C = buf + c0; g.sC = stride; g.ldc = W; g.tC = 0;
-        g.alpha = -1.0; g.beta = 1.0; g.batch = batch;
-        e = launch_gemm<double>(g, tile, s);
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    return trsm_range(buf, stride, N, W, c0, P, batch, tile, 0, (N + TRB - 1) / TRB, s);
 }
 
 // ---------------------------------------------------------------------------------------------
