@@ -121,6 +121,12 @@ struct dqmc_ctx {
     double* ident = nullptr;               // one N x N identity, read at batch stride 0
     DStack* chain_stack = nullptr;         // the stack the current B-chain block is pushed onto
     bool chain_first = false;              // no factor of the block applied yet
+    // B-chain factors on the 16 x 16 lattice are collected and applied in one pass at the push
+    // (DQMC_CHAIN=slice applies each one when its slice is done)
+    bool chain_fused = true;
+    int chain_pend[KCHAIN_MAX] = {};
+    int chain_npend = 0;
+    bool chain_pend_adj = false;
     double *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *qr = nullptr;
     int* perm = nullptr;
     double *gt0A = nullptr, *gt0B = nullptr, *g0tA = nullptr, *g0tB = nullptr, *gt0 = nullptr, *g0t = nullptr;
@@ -436,22 +442,61 @@ void chain_begin(dqmc_ctx* c, DStack& s)
     c->stk_alt = c->stkB;
     c->chain_stack = &s;
     c->chain_first = true;
+    c->chain_npend = 0;
 }
 // One factor of the block's B-chain: stk <- B(t) stk (src/dqmc_core.cpp:190) or B(t)^T stk (:251, adjB,
 // src/hubbard.cpp:333-341).  The first factor of a block reads the top U of the stack the block goes to
 // (the identity when that stack is empty) instead of an identity-initialized chain, and the last one
 // (`last`) scales the columns by that stack's top D, so the push receives (chain * U_top) * diag(D_top)
 // (svd_stack.hpp:144) with no separate N x N x N product.
+// The operand of the block's next factor: the chain so far, or the stack top's U (the identity at
+// batch stride 0 for an empty stack) before the first factor.
+const double* chain_operand(dqmc_ctx* c, long long& sX)
+{
+    DStack& s = *c->chain_stack;
+    const long long nn = c->NN();
+    sX = nn;
+    if (!c->chain_first) return c->stk;
+    if (s.len > 0) { sX = (long long)c->S * nn; return sU(c, s, s.len - 1); }
+    sX = 0;
+    return c->ident;
+}
+// Apply the collected factors of the block in one pass (16 x 16 lattice, FP64 chain).
+int chain_flush(dqmc_ctx* c, bool last)
+{
+    if (c->chain_npend == 0) return 0;
+    DStack& s = *c->chain_stack;
+    KronChainArgs k;
+    k.X = chain_operand(c, k.sX);
+    k.C = c->stk_alt; k.sC = c->NN();
+    k.E = c->E1;
+    k.fields = c->fields; k.fstride = (long long)c->L * c->N;
+    k.table = c->tab;
+    for (int i = 0; i < c->chain_npend; ++i) k.ts[i] = c->chain_pend[i];
+    k.n = c->chain_npend;
+    k.adjoint = c->chain_pend_adj ? 1 : 0;
+    k.cold = (last && s.len > 0) ? dvec(DIAG_VEC, sD(c, s, s.len - 1), (long long)c->S * c->N) : dnone();
+    k.batch = c->B;
+    TIMED(c, KT_KRON, launch_kron_chain16(k, c->stream));
+    std::swap(c->stk, c->stk_alt);
+    c->chain_first = false;
+    c->chain_npend = 0;
+    return 0;
+}
 int chain_mul(dqmc_ctx* c, int t, bool adjoint, bool last)
 {
     DStack& s = *c->chain_stack;
     const long long nn = c->NN();
-    const double* X = c->stk;
-    long long sX = nn;
-    if (c->chain_first) {
-        if (s.len > 0) { X = sU(c, s, s.len - 1); sX = (long long)c->S * nn; }
-        else { X = c->ident; sX = 0; }
+    if (c->chain_fused && c->kron && c->nl == 16) {
+        // the factors of a block are applied when it is pushed, after its last slice's update
+        if (c->chain_npend > 0 && c->chain_pend_adj != adjoint) RCHECK(chain_flush(c, false));
+        c->chain_pend[c->chain_npend++] = t;
+        c->chain_pend_adj = adjoint;
+        if (last || c->chain_npend == KCHAIN_MAX) RCHECK(chain_flush(c, last));
+        return 0;
     }
+    long long sX = nn;
+    const double* X = chain_operand(c, sX);
     const DiagArg d = dfield(c, c->tab, t);
     const DiagArg rowd = adjoint ? dnone() : d, innd = adjoint ? d : dnone();
     const DiagArg cold = (last && s.len > 0) ? dvec(DIAG_VEC, sD(c, s, s.len - 1), (long long)c->S * c->N) : dnone();
@@ -904,6 +949,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         c->trsm_v1 = tr && std::string(tr) == "v1";
         const char* w = std::getenv("DQMC_WRAP");
         c->kron = !(w && std::string(w) == "dense");
+        const char* ch = std::getenv("DQMC_CHAIN");
+        c->chain_fused = !(ch && std::string(ch) == "slice");
         if (N > 256) {
             // Lattices above 16 x 16: factored slice products on the 32 x 32 padded ring factor in
             // FP64 (dense GEMMs for MIXED), the Householder QR solve with the wide panel kernel and

@@ -702,6 +702,95 @@ __global__ __launch_bounds__(512) void kron_right16_kernel(KronArgs<T> a)
     }
 }
 
+// Several B-chain factors on the 16 x 16 lattice in one pass (FP64).  The f64 accumulator item r of
+// lane l is Z(x = 4 r + l/16, y = l%16), which is input slot s = r of the transposed vector
+// Z^T(x' = l%16, y' = l/16 + 4 s).  (E(x)E) acts alike on both lattice directions, so the next factor
+// applied to Z^T gives the transpose of its product with Z: each factor takes the previous one's
+// accumulators as its input with no data movement.  Only the site of a slot alternates between the
+// two forms, and the diagonal scalings follow it:
+//   normal     (V):   slot (l, s) is site l + 64 s
+//   transposed (V^T): slot (l, s) is site l/16 + 4 s + 16 (l%16)
+// The result is stored directly in the normal form and through a padded LDS transpose in the
+// transposed one.  Per element each factor does what one kron_left16_kernel launch does
+// (inner scaling, the two 16 x 16 x 16 products, then (z * cv) * row scaling); the sums of the
+// products run over the lattice directions in the other order in the transposed form, so the
+// result equals the launch-per-factor one to rounding.  Each wave carries 4 columns of X.
+__global__ __launch_bounds__(256) void kron_chain16_kernel(KronChainArgs a)
+{
+    __shared__ double zt[4][16][17];
+    const int b = blockIdx.y, lane = threadIdx.x & 63, wave = threadIdx.x >> 6, l16 = lane & 15, q = lane >> 4;
+    double ef[4], ef2[4];
+    load_efrag<double>(a.E, lane, ef, ef2);
+    const double* X = a.X + (long long)b * a.sX;
+    double* C = a.C + (long long)b * a.sC;
+    const int8_t* F = a.fields + (long long)b * a.fstride;
+    const double dp = a.table[2 * b], dm = a.table[2 * b + 1];
+    const int cbase = blockIdx.x * 16 + wave * 4;
+    double v[4][4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) v[cc][s] = X[(long long)(cbase + cc) * 256 + lane + 64 * s];
+    const int n = a.n;
+    bool tr = false;   // form of the current vectors
+    for (int k = 0; k < n; ++k) {
+        const int8_t* f = F + (long long)a.ts[k] * 256;
+        if (a.adjoint) {   // B^T = (E(x)E) diag(d): scale the input sites
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int site = tr ? (q + 4 * s + 16 * l16) : (lane + 64 * s);
+                const double d = f[site] > 0 ? dp : dm;
+#pragma unroll
+                for (int cc = 0; cc < 4; ++cc) v[cc][s] = v[cc][s] * d;
+            }
+        }
+        d4_t z[4];
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) kron16_vec<double>(ef, ef2, v[cc], z[cc]);
+        tr = !tr;
+        double dout[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int site = tr ? (q + 4 * r + 16 * l16) : (lane + 64 * r);
+            dout[r] = a.adjoint ? 1.0 : (f[site] > 0 ? dp : dm);
+        }
+        const bool last = k == n - 1;
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            const double cs = (last && a.cold.mode != DIAG_NONE) ? diag_value(a.cold, b, cbase + cc) : 1.0;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[cc][r] = (z[cc][r] * cs) * dout[r];
+        }
+    }
+    if (!tr) {
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) C[(long long)(cbase + cc) * 256 + lane + 64 * r] = v[cc][r];
+    } else {
+        // slot (l, r) holds site (4 r + l/16) + 16 (l%16): back to consecutive sites through LDS
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) zt[wave][l16][4 * r + q] = v[cc][r];
+            wave_lds_sync();
+            double zv[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) zv[s] = zt[wave][q + 4 * s][l16];
+            wave_lds_sync();   // read before the next column overwrites the buffer
+#pragma unroll
+            for (int s = 0; s < 4; ++s) C[(long long)(cbase + cc) * 256 + lane + 64 * s] = zv[s];
+        }
+    }
+}
+
+hipError_t launch_kron_chain16(const KronChainArgs& a, hipStream_t s)
+{
+    if (a.n < 1 || a.n > KCHAIN_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(kron_chain16_kernel, dim3(16, a.batch), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_kron_left(const KronArgs<T>& a, hipStream_t s)
 {

@@ -63,6 +63,24 @@ struct KronArgs {
     DiagArg rowd, innd, cold;
     int batch;
 };
+// Several factors of a B-chain block in one pass over the 16 x 16 lattice (FP64):
+//   C = B(ts[n-1]) .. B(ts[0]) X diag(cv)                   (adjoint = 0, B(t) = diag(d_t) (E(x)E))
+//   C = B(ts[n-1])^T .. B(ts[0])^T X diag(cv)               (adjoint = 1, B(t)^T = (E(x)E) diag(d_t))
+// with d_t(i) = table[2 b] when the field of slice t at site i is positive, else table[2 b + 1].
+// Each lattice vector (column of X) stays in registers over all n factors.
+#define KCHAIN_MAX 16
+struct KronChainArgs {
+    const double* X; long long sX;
+    double* C; long long sC;
+    const double* E;                 // 16 x 16, row-major, padded
+    const int8_t* fields; long long fstride;   // fields[b * fstride + t * 256 + i]
+    const double* table;             // 2 values per chain
+    int ts[KCHAIN_MAX];
+    int n, adjoint;
+    DiagArg cold;
+    int batch;
+};
+hipError_t launch_kron_chain16(const KronChainArgs& a, hipStream_t s);
 // Host: the message of the calling thread's last failed call without a context
 // (dqmc_last_error(NULL)); free_error records it and returns code.
 int free_error(int code, const char* msg);

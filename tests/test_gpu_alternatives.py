@@ -20,6 +20,9 @@ one exercised).
   DQMC_METRO_THREADS     Metropolis workgroup of 256 or 512 threads instead of 1024
   DQMC_LOOKAHEAD=0       blocked QR and LU panels and updates in order on one stream, instead of
                          each next panel beside the rest of the previous update on a second stream
+  DQMC_CHAIN=slice       each B-chain factor applied when its slice is done, instead of all factors
+                         of a stack block in one pass at its push (also checked at pace 17, where
+                         a block takes two passes of at most 16 factors)
 """
 import sys
 
@@ -39,8 +42,8 @@ def say(*a):
     sys.stdout.flush()
 
 
-def run_chains(n_sweeps):
-    e = gpu.Engine(NL, NT, BETA, U, mu=MU, pace=PACE, n_chains=NCH, delay_k=K)
+def run_chains(n_sweeps, pace=PACE, beta=BETA):
+    e = gpu.Engine(NL, NT, beta, U, mu=MU, pace=pace, n_chains=NCH, delay_k=K)
     for c in range(NCH):
         e.set_seed(c, 4100 + c)
         e.fields_random(c)
@@ -62,17 +65,32 @@ def default_run():
 
 @pytest.mark.parametrize("var,val", [("DQMC_GEMM_TILE", "1"), ("DQMC_GEMM_TILE", "3"), ("DQMC_GEMM_TILE", "4"), ("DQMC_SOLVE", "qr"), ("DQMC_QR_PANEL", "1024"), ("DQMC_QR_UPDATE", "gemm"), ("DQMC_WRAP", "dense"), ("DQMC_TRSM", "v1"),
                                      ("DQMC_QR", "v1"), ("DQMC_METRO_THREADS", "256"),
-                                     ("DQMC_METRO_THREADS", "512"), ("DQMC_LOOKAHEAD", "0")])
+                                     ("DQMC_METRO_THREADS", "512"), ("DQMC_LOOKAHEAD", "0"),
+                                     ("DQMC_CHAIN", "slice")])
 def test_switch_follows_default_chains(default_run, monkeypatch, var, val):
     say("case: %s=%s, 16x16 L=40, 8 chains, 2 sweeps" % (var, val))
     monkeypatch.setenv(var, val)
     alt, wrap = run_chains(2)
-    ref, ref_wrap = default_run
-    assert ref_wrap < 1e-8 and wrap < 1e-8
+    compare_runs(alt, wrap, *default_run)
+
+
+def test_chain_block_longer_than_one_pass(monkeypatch):
+    # blocks of 17, 17 and 6 slices at beta = 1 (dt = 0.025), where wrap intervals of 17 slices keep
+    # the FP64 wrap error as small as pace 8 does at beta = 4
+    say("case: pace 17 (B-chain blocks of 17 factors), beta 1, fused against DQMC_CHAIN=slice, 16x16 L=40, 8 chains")
+    fused, wf = run_chains(2, pace=17, beta=1.0)
+    monkeypatch.setenv("DQMC_CHAIN", "slice")
+    per_slice, ws = run_chains(2, pace=17, beta=1.0)
+    say("  wrap errors: fused %.2e, per slice %.2e" % (wf, ws))
+    compare_runs(fused, wf, per_slice, ws)
+
+
+def compare_runs(alt, wrap, ref, ref_wrap, wrap_max=1e-8, tol=TOL_G):
+    assert ref_wrap < wrap_max and wrap < wrap_max
     for sweep, ((ma, ga), (mr, gr)) in enumerate(zip(alt, ref)):
         for c in range(NCH):
             assert np.array_equal(ma[c], mr[c]), "chain %d sweep %d: accept decisions differ" % (c, sweep)
             d = np.abs(ga[c] - gr[c]).max()
-            assert d < TOL_G, "chain %d sweep %d: max|dG| %g" % (c, sweep, d)
+            assert d < tol, "chain %d sweep %d: max|dG| %g" % (c, sweep, d)
         say("  sweep %d: decisions identical, max|dG| = %.2e" %
             (sweep, max(np.abs(ga[c] - gr[c]).max() for c in range(NCH))))
