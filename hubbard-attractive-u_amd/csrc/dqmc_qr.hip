@@ -881,6 +881,16 @@ __device__ __forceinline__ void argmax_take(double& v, int& i, double ov, int oi
 {
     if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
 }
+// max of v over the 64 lanes, in every lane (max is exact, so the order of the pairings is free)
+__device__ __forceinline__ double wave_max_f64(double v, int lane)
+{
+    v = fmax(v, dpp_f64<DPP_XOR1>(v));
+    v = fmax(v, dpp_f64<DPP_XOR2>(v));
+    v = fmax(v, dpp_f64<DPP_HALF_MIRROR>(v));
+    v = fmax(v, dpp_f64<DPP_MIRROR>(v));
+    v = fmax(v, xchg_f64(v, lane, true));
+    return fmax(v, xchg_f64(v, lane, false));
+}
 // (value, index) arg-max over the 64 lanes; every lane ends with the result.  The pairings (quad
 // perms, half-row and row mirrors, row-pair and half swaps) each join two groups that already
 // agree, so after the six levels the whole wave agrees.
@@ -931,10 +941,15 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs
     for (int i = 0; i < QRB; ++i) {
         if (i >= nb) continue;   // uniform
         const int par = i & 1;
-        double cv = (live && r >= i) ? fabs(x[i]) : -1.0;
-        int ci = r;
-        wave_argmax(cv, ci, lane);
-        if (r == ci) {   // this lane holds its wave's candidate row
+        // the wave's candidate: the largest |x(r, i)|, ties to the lowest row (lowest lane of the
+        // ballot); a max reduction and a ballot are a shorter dependent chain than a (value, index)
+        // arg-max
+        const double cl = (live && r >= i) ? fabs(x[i]) : -1.0;
+        const double cv = wave_max_f64(cl, lane);
+        const unsigned long long hit = __ballot(cl == cv);
+        const int lmin = hit ? __builtin_ctzll(hit) : 0;
+        const int ci = wave * 64 + lmin;
+        if (lane == lmin) {   // this lane holds its wave's candidate row
 #pragma unroll
             for (int m = 0; m < QRB; m += 2) {
                 double2 v; v.x = x[m]; v.y = x[m + 1];
