@@ -127,6 +127,10 @@ struct dqmc_ctx {
     int chain_pend[KCHAIN_MAX] = {};
     int chain_npend = 0;
     bool chain_pend_adj = false;
+    // FP64 16 x 16 up sweeps: the Metropolis workgroup of slice t also applies the left product of the
+    // wrap to slice t + 1 (into W1) when no stabilization comes between (DQMC_WRAP_FUSE=0 turns it off)
+    bool wrap_fuse = true;
+    bool wrap_left_done = false;
     double *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *qr = nullptr;
     int* perm = nullptr;
     double *gt0A = nullptr, *gt0B = nullptr, *g0tA = nullptr, *g0tB = nullptr, *gt0 = nullptr, *g0t = nullptr;
@@ -348,7 +352,9 @@ int wrap_up(dqmc_ctx* c, int t)
             RCHECK(kron<float>(c, true, c->G32, c->T32, c->E1f, dfield(c, c->tab, t)));
             RCHECK(kron<float>(c, false, c->T32, c->G32, c->F1f, dnone(), dnone(), dfield(c, c->itab, t)));
         } else {
-            RCHECK(kron<double>(c, true, c->G64, c->W1, c->E1, dfield(c, c->tab, t)));
+            // W1 = D (E(x)E) G is already there when the previous slice's Metropolis workgroups made it
+            if (!c->wrap_left_done) RCHECK(kron<double>(c, true, c->G64, c->W1, c->E1, dfield(c, c->tab, t)));
+            c->wrap_left_done = false;
             RCHECK(kron<double>(c, false, c->W1, c->G64, c->F1, dnone(), dnone(), dfield(c, c->itab, t)));
         }
         return 0;
@@ -410,7 +416,15 @@ int right_expk(dqmc_ctx* c, const double* X, double* C, const double* table, int
                         inner ? d : dnone(), inner ? dnone() : d);
 }
 // stacks <- B(t) stacks (src/dqmc_core.cpp:190) or B(t)^T stacks (:251, adjB, src/hubbard.cpp:333-341)
-int metropolis(dqmc_ctx* c, int t)
+// Whether the Metropolis workgroups can also make the next wrap's left product (FP64 Kronecker
+// products on the 16 x 16 lattice).
+bool wrap_fusable(dqmc_ctx* c)
+{
+    return c->wrap_fuse && c->kron && c->prec == DQMC_FP64 && c->nl == 16;
+}
+// Metropolis update of slice t; with tnext >= 0 the workgroups then form W1 = D(tnext) (E(x)E) G, the
+// left product of wrap_up(tnext), which then applies only its right product.
+int metropolis(dqmc_ctx* c, int t, int tnext = -1)
 {
     if (c->prec == DQMC_MIXED) {
         MetroArgs<float> a;
@@ -425,7 +439,11 @@ int metropolis(dqmc_ctx* c, int t)
         a.fields = c->fields; a.N = c->N; a.L = c->L; a.t = t;
         a.rng = c->rng; a.powtab = c->powtab; a.delta = c->delta;
         a.accmask = c->accmask; a.acc_count = c->acc_count; a.health = c->health; a.kmax = c->kmax;
+        if (tnext >= 0) {
+            a.lout = c->W1; a.lE = c->E1; a.ltab = c->tab; a.lt = tnext; a.lmode = 1;
+        }
         TIMED(c, KT_METRO, launch_metropolis<double>(a, c->B, c->metro_threads, c->stream));
+        c->wrap_left_done = tnext >= 0;
     }
     return 0;
 }
@@ -631,11 +649,14 @@ int do_sweep_up(dqmc_ctx* c)
 {
     const int L = c->L;
     const bool store = !c->thermalization && c->eqtime;
+    const bool fuse = wrap_fusable(c);
+    c->wrap_left_done = false;
     chain_begin(c, c->left);
     for (int t = 1; t <= L; ++t) {
         const bool push = t % c->pace == 0 || t == L;
         RCHECK(wrap_up(c, t % L));
-        RCHECK(metropolis(c, t % L));
+        // unless a stabilization follows, the next step is wrap_up(t + 1)
+        RCHECK(metropolis(c, t % L, (fuse && !push) ? (t + 1) % L : -1));
         RCHECK(chain_mul(c, t % L, false, push));
         if (push) {
             c->right.len -= 1;
@@ -949,6 +970,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         c->trsm_v1 = tr && std::string(tr) == "v1";
         const char* w = std::getenv("DQMC_WRAP");
         c->kron = !(w && std::string(w) == "dense");
+        const char* wf = std::getenv("DQMC_WRAP_FUSE");
+        c->wrap_fuse = !(wf && std::string(wf) == "0");
         const char* ch = std::getenv("DQMC_CHAIN");
         c->chain_fused = !(ch && std::string(ch) == "slice");
         if (N > 256) {

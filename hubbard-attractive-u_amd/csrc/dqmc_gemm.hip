@@ -444,19 +444,6 @@ __device__ __forceinline__ void kron_tile(const T (&ef)[4], const T (&ef2)[4], i
     }
 }
 
-// E fragments B(k, col) = E[col][k], col = l%16.  Step 1 contracts k = l/16 + 4s.  Step 2
-// contracts over the rows of step 1's accumulator, item s of lane l: k = Mfma<T>::row(l, s),
-// i.e. 4s + l/16 for f64 and 4(l/16) + s for f32.
-template <typename T>
-__device__ __forceinline__ void load_efrag(const T* E, int lane, T (&ef)[4], T (&ef2)[4])
-{
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        ef[s] = E[(lane & 15) * 16 + (lane >> 4) + 4 * s];
-        ef2[s] = E[(lane & 15) * 16 + Mfma<T>::row(lane, s)];
-    }
-}
-
 template <typename T>
 __global__ __launch_bounds__(256) void kron_left_kernel(KronArgs<T> a)
 {
@@ -563,25 +550,6 @@ __global__ __launch_bounds__(256) void kron_right_kernel(KronArgs<T> a)
             C[row + (long long)c * N] = T(v);
         }
     }
-}
-
-// 16 x 16 lattice (N = 256): a lattice vector read as 64 lanes x 4 consecutive sites,
-// site = l + 64 s, is exactly the A-fragment V(x = l%16, y = 4 s + l/16) of step 1, so
-//   step 1  Y = V E^T   (A = V from a coalesced load, B(y, j) = E(j, y))
-//   step 2  Z = E Y     (A(i, x) = E(i, x), B = step 1's accumulator items)
-// and Z = E V E^T leaves the accumulator as Z(row(l, r), l%16), which a padded LDS transpose turns
-// back into consecutive sites for a coalesced store.
-template <typename T>
-__device__ __forceinline__ void kron16_vec(const T (&ef)[4], const T (&ef2)[4], const T (&v)[4],
-                                           typename Mfma<T>::acc_t& z)
-{
-    typedef typename Mfma<T>::acc_t acc_t;
-    acc_t y = acc_t{0, 0, 0, 0};
-#pragma unroll
-    for (int s = 0; s < 4; ++s) y = Mfma<T>::mma(v[s], ef[s], y);
-    z = acc_t{0, 0, 0, 0};
-#pragma unroll
-    for (int s = 0; s < 4; ++s) z = Mfma<T>::mma(ef2[s], y[s], z);
 }
 
 // Left product on the 16 x 16 lattice: each wave transforms 4 columns of X, one after the other

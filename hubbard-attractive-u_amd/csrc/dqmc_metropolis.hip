@@ -355,6 +355,51 @@ __device__ __forceinline__ void rebuild_chain(double (&x)[MB_MAX], int na, const
     ((go = go && (M < na), go ? (rebuild_one<SCALE, M>(x, na, coef, qa, qb, gam, ca, cb, g), 0) : 0), ...);
 }
 
+// The next wrap's left product by the workgroup of chain b after its slice (FP64, N = 256): each wave
+// transforms 2 columns at a time as kron_left16_kernel does (the same per-element arithmetic), through
+// its own 16 x 17 transpose buffer at the start of the LDS, which the pending update no longer uses.
+template <int MT>
+__device__ __forceinline__ void metro_left16(const MetroArgs<double>& a, int b, unsigned char* smem)
+{
+    double(*zt)[16][17] = reinterpret_cast<double(*)[16][17]>(smem);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int NWV = MT / 64, CG = 2;   // columns in flight per wave (the kernel's register budget)
+    double ef[4], ef2[4];
+    load_efrag<double>(a.lE, lane, ef, ef2);
+    const double* G = a.G + (long long)b * a.sG;
+    double* C = a.lout + (long long)b * a.sG;
+    const int8_t* f = a.fields + ((long long)b * a.L + a.lt) * 256;
+    const double dp = a.ltab[2 * b], dm = a.ltab[2 * b + 1];
+    double ds[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) ds[s] = f[lane + 64 * s] > 0 ? dp : dm;
+    const bool rows = a.lmode == 1;
+    for (int c0 = wave * CG; c0 < 256; c0 += NWV * CG) {
+        double vin[CG][4];
+#pragma unroll
+        for (int cc = 0; cc < CG; ++cc)
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const double v = G[(long long)(c0 + cc) * 256 + lane + 64 * s];
+                vin[cc][s] = rows ? v : v * ds[s];
+            }
+#pragma unroll
+        for (int cc = 0; cc < CG; ++cc) {
+            d4_t z;
+            kron16_vec<double>(ef, ef2, vin[cc], z);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) zt[wave][lane & 15][Mfma<double>::row(lane, r)] = z[r];
+            wave_lds_sync();
+            double zv[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) zv[s] = zt[wave][(lane >> 4) + 4 * s][lane & 15];
+            wave_lds_sync();   // read before the next column overwrites the buffer
+#pragma unroll
+            for (int s = 0; s < 4; ++s) C[(long long)(c0 + cc) * 256 + lane + 64 * s] = rows ? zv[s] * ds[s] : zv[s];
+        }
+    }
+}
+
 // Submatrix Metropolis update of slice t.  Sites are visited in blocks S of kb consecutive
 // sites.  Inside a block the accept/reject sequence runs on Gs = G(S,S) in LDS with the
 // reference's rank-1 update.  After the block the accepted flips A are applied to the whole
@@ -494,6 +539,10 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
         a.rng[b] = (x0 * a.powtab[2 * N - 1]) % MINSTD_M;
         a.acc_count[b] += (unsigned long long)nacc;
     }
+    if constexpr (sizeof(T) == 8) {
+        // the final flush's stores of G are visible to the whole workgroup past the barrier above
+        if (a.lmode != 0) metro_left16<MT>(a, b, smem);
+    }
 }
 
 
@@ -514,7 +563,10 @@ template <typename T>
 hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, int threads, hipStream_t s)
 {
     if (a.kmax < 1 || a.kmax > MB_MAX) return hipErrorInvalidValue;
-    const size_t shm = metropolis_lds_bytes(a.N, a.kmax, sizeof(T));
+    if (a.lmode != 0 && (sizeof(T) != 8 || a.N != 256 || !a.lout || !a.lE || !a.ltab || a.lt < 0 || a.lt >= a.L))
+        return hipErrorInvalidValue;
+    size_t shm = metropolis_lds_bytes(a.N, a.kmax, sizeof(T));
+    if (a.lmode != 0 && shm < (size_t)16 * 16 * 17 * 8) shm = (size_t)16 * 16 * 17 * 8;   // the epilogue's transpose buffers
     if (shm > 160 * 1024) return hipErrorInvalidValue;
     if (a.N < 256) threads = ((a.N + 63) / 64) * 64 * 2 <= 1024 ? ((a.N + 63) / 64) * 64 * 2 : 1024;
     if (threads <= 256)

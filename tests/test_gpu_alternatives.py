@@ -20,6 +20,8 @@ one exercised).
   DQMC_METRO_THREADS     Metropolis workgroup of 256 or 512 threads instead of 1024
   DQMC_LOOKAHEAD=0       blocked QR and LU panels and updates in order on one stream, instead of
                          each next panel beside the rest of the previous update on a second stream
+  DQMC_WRAP_FUSE=0       every wrap's left product by its own launch, instead of by the Metropolis
+                         workgroups of the slice before (up sweeps without a stabilization between)
   DQMC_CHAIN=slice       each B-chain factor applied when its slice is done, instead of all factors
                          of a stack block in one pass at its push (also checked at pace 17, where
                          a block takes two passes of at most 16 factors)
@@ -66,7 +68,7 @@ def default_run():
 @pytest.mark.parametrize("var,val", [("DQMC_GEMM_TILE", "1"), ("DQMC_GEMM_TILE", "3"), ("DQMC_GEMM_TILE", "4"), ("DQMC_SOLVE", "qr"), ("DQMC_QR_PANEL", "1024"), ("DQMC_QR_UPDATE", "gemm"), ("DQMC_WRAP", "dense"), ("DQMC_TRSM", "v1"),
                                      ("DQMC_QR", "v1"), ("DQMC_METRO_THREADS", "256"),
                                      ("DQMC_METRO_THREADS", "512"), ("DQMC_LOOKAHEAD", "0"),
-                                     ("DQMC_CHAIN", "slice")])
+                                     ("DQMC_CHAIN", "slice"), ("DQMC_WRAP_FUSE", "0")])
 def test_switch_follows_default_chains(default_run, monkeypatch, var, val):
     say("case: %s=%s, 16x16 L=40, 8 chains, 2 sweeps" % (var, val))
     monkeypatch.setenv(var, val)
