@@ -445,7 +445,6 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
     __syncthreads();
     const int lane = tid & 63, wave = tid >> 6, nwv = nth >> 6, l16 = lane & 15;
     for (int b0 = 0; b0 < N; b0 += kb) {
-        const int bs = min(kb, N - b0);
         // The count of block b0 sits in its own parity slot: when na == 0 no barrier separates this
         // read from the site loop of the next block, which writes the other slot.
         const int na = L.nacc[(b0 / kb) & 1];
@@ -455,37 +454,14 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
         const int tr0 = next ? b1 / 16 : 0, tr1 = next ? (b1 + bs1 - 1) / 16 : 0;
         const int a4 = ((na + 3) / 4) * 4;
         if (na > 0) {
-            // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  All waves first stage
-            // the block's rows G0(S, :) in Yl (16 lanes per column, two rows per lane: contiguous
-            // global reads, all in flight at once).  Then each wave takes whole 64-column tasks, X
-            // or Y, and runs the recursion of its columns in registers with all 64 lanes active (the
-            // coefficients travel by DPP broadcast, which needs every source lane).
+            // Rebuild X (columns) and Y (rows) of the rank-na update from G0.  Each wave takes whole
+            // 64-column tasks, X or Y, reads the G0 entries of its columns (a column segment of G for X,
+            // one element of each of its 64 columns per row for Y) and runs the recursion of its columns
+            // in registers with all 64 lanes active (the coefficients travel by DPP broadcast, which
+            // needs every source lane).
             const int ys = L.ys, nct = (N + 63) / 64;
             const int qlane = (lane < na) ? L.ql[lane] : 0;   // lane m holds q_m
             const int qa = (l16 < na) ? L.ql[l16] : MB_MAX, qb = (16 + l16 < na) ? L.ql[16 + l16] : MB_MAX;
-            {
-                constexpr int SI = (16 * 256 + MT - 1) / MT;   // staging items per thread at N <= 256
-                const int r = 2 * l16, r0 = min(r, bs - 1), r1 = min(r + 1, bs - 1);
-                T v0[SI], v1[SI];
-#pragma unroll
-                for (int i = 0; i < SI; ++i) {
-                    const int jj = min(((tid + i * nth) >> 4), N - 1);
-                    v0[i] = G[(b0 + r0) + (long long)jj * N];
-                    v1[i] = G[(b0 + r1) + (long long)jj * N];
-                }
-#pragma unroll
-                for (int i = 0; i < SI; ++i) {
-                    const int jj = (tid + i * nth) >> 4;
-                    if (jj < N && r < bs) L.Yl[r * ys + (jj ^ yswz(r))] = v0[i];
-                    if (jj < N && r + 1 < bs) L.Yl[(r + 1) * ys + (jj ^ yswz(r + 1))] = v1[i];
-                }
-                for (int w = tid + SI * nth; w < N * 16; w += nth) {   // workgroups narrower than MT
-                    const int jj = w >> 4;
-                    if (r < bs) L.Yl[r * ys + (jj ^ yswz(r))] = G[(b0 + r) + (long long)jj * N];
-                    if (r + 1 < bs) L.Yl[(r + 1) * ys + (jj ^ yswz(r + 1))] = G[(b0 + r + 1) + (long long)jj * N];
-                }
-            }
-            __syncthreads();
             for (int task = wave; task < 2 * nct; task += nwv) {
                 const bool isx = task < nct;
                 const int j = (isx ? task : task - nct) * 64 + lane;
@@ -505,7 +481,7 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
 #pragma unroll
                     for (int m = 0; m < MB_MAX; ++m) {
                         const int q = __builtin_amdgcn_readlane(qlane, m);
-                        xr[m] = (m < na && jin) ? (double)L.Yl[q * ys + (j ^ yswz(q))] - (j == b0 + q ? 1.0 : 0.0) : 0.0;
+                        xr[m] = (m < na && jin) ? (double)G[(b0 + q) + (long long)j * N] - (j == b0 + q ? 1.0 : 0.0) : 0.0;
                     }
                     rebuild_chain<false>(xr, na, L.XS, qa, qb, L.gam, std::make_integer_sequence<int, MB_MAX>{});
 #pragma unroll
