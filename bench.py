@@ -326,13 +326,14 @@ def main():
     avg_s = ms / n / 1e3
     # Algorithmic work of one launch (one slice of every chain): one read and one write of each
     # chain's N x N FP64 G, and the accepted flips' rank-1 Sherman-Morrison updates, 2 N^2 flops
-    # each (src/hubbard.cpp:132-139).  In an up sweep the launches of the slices not followed by a
-    # stabilization (NT - S of the NT) also form the next wrap's left product, exp(-dt K) G: one more
-    # read of G and one write of an N x N result, and 2 (16^3 + 16^3) flops per column with the
-    # Kronecker factors.  The two timed sweeps are one up and one down sweep.  The kernel moves
-    # several times the algorithmic bytes (the flushes of its site blocks), so HBM is what bounds it.
+    # each (src/hubbard.cpp:132-139).  The launches also form a wrap's left product, exp(-+dt K) G,
+    # in an up sweep for the slices not followed by a stabilization (NT - S of the NT) and in a down
+    # sweep for every slice: one more read of G and one write of an N x N result, and 2 (16^3 + 16^3)
+    # flops per column with the Kronecker factors.  The two timed sweeps are one up and one down
+    # sweep.  The kernel moves several times the algorithmic bytes (the flushes of its site blocks),
+    # so HBM is what bounds it.
     S = -(-NT // PACE)
-    fused = (NT - S) if os.environ.get("DQMC_WRAP_FUSE", "") != "0" else 0
+    fused = (2 * NT - S) if os.environ.get("DQMC_WRAP_FUSE", "") != "0" else 0
     ffrac = fused / float(n)
     abytes = 2.0 * N * N * 8 * args.chains * (1.0 + ffrac)
     flops = 2.0 * N * N * accs / n + ffrac * 4.0 * 16 ** 3 * N * args.chains

@@ -380,6 +380,12 @@ int wrap_down(dqmc_ctx* c, int t)
         if (c->prec == DQMC_MIXED) {
             RCHECK(kron<float>(c, false, c->G32, c->T32, c->E1f, dnone(), dfield(c, c->tab, t)));
             RCHECK(kron<float>(c, true, c->T32, c->G32, c->F1f, dnone(), dfield(c, c->itab, t)));
+        } else if (c->wrap_left_done) {
+            // W1 = (F(x)F) D^{-1} G was formed by the slice's Metropolis workgroups: the right product
+            // follows the left one here (the reference applies B from the right first; the two orders
+            // agree to rounding)
+            c->wrap_left_done = false;
+            RCHECK(kron<double>(c, false, c->W1, c->G64, c->E1, dnone(), dfield(c, c->tab, t)));
         } else {
             RCHECK(kron<double>(c, false, c->G64, c->W1, c->E1, dnone(), dfield(c, c->tab, t)));
             RCHECK(kron<double>(c, true, c->W1, c->G64, c->F1, dnone(), dfield(c, c->itab, t)));
@@ -415,16 +421,17 @@ int right_expk(dqmc_ctx* c, const double* X, double* C, const double* table, int
     return gemm<double>(c, X, nn, c->N, 0, fwd ? c->expK : c->invExpK, 0, c->N, 0, C, nn, c->N, 0, dnone(),
                         inner ? d : dnone(), inner ? dnone() : d);
 }
-// stacks <- B(t) stacks (src/dqmc_core.cpp:190) or B(t)^T stacks (:251, adjB, src/hubbard.cpp:333-341)
 // Whether the Metropolis workgroups can also make the next wrap's left product (FP64 Kronecker
 // products on the 16 x 16 lattice).
 bool wrap_fusable(dqmc_ctx* c)
 {
     return c->wrap_fuse && c->kron && c->prec == DQMC_FP64 && c->nl == 16;
 }
-// Metropolis update of slice t; with tnext >= 0 the workgroups then form W1 = D(tnext) (E(x)E) G, the
-// left product of wrap_up(tnext), which then applies only its right product.
-int metropolis(dqmc_ctx* c, int t, int tnext = -1)
+// Metropolis update of slice t.  With a fused wrap the workgroups then form the left product of the next
+// wrap into W1, and that wrap applies only its right product: FUSE_UP forms D(t') (E(x)E) G for
+// wrap_up(t') with t' = lt, FUSE_DOWN forms (F(x)F) D(t)^{-1} G for wrap_down(t).
+enum WrapFuse { FUSE_NONE = 0, FUSE_UP = 1, FUSE_DOWN = 2 };
+int metropolis(dqmc_ctx* c, int t, int fuse = FUSE_NONE, int lt = 0)
 {
     if (c->prec == DQMC_MIXED) {
         MetroArgs<float> a;
@@ -439,11 +446,13 @@ int metropolis(dqmc_ctx* c, int t, int tnext = -1)
         a.fields = c->fields; a.N = c->N; a.L = c->L; a.t = t;
         a.rng = c->rng; a.powtab = c->powtab; a.delta = c->delta;
         a.accmask = c->accmask; a.acc_count = c->acc_count; a.health = c->health; a.kmax = c->kmax;
-        if (tnext >= 0) {
-            a.lout = c->W1; a.lE = c->E1; a.ltab = c->tab; a.lt = tnext; a.lmode = 1;
+        if (fuse == FUSE_UP) {
+            a.lout = c->W1; a.lE = c->E1; a.ltab = c->tab; a.lt = lt; a.lmode = 1;
+        } else if (fuse == FUSE_DOWN) {
+            a.lout = c->W1; a.lE = c->F1; a.ltab = c->itab; a.lt = t; a.lmode = 2;
         }
         TIMED(c, KT_METRO, launch_metropolis<double>(a, c->B, c->metro_threads, c->stream));
-        c->wrap_left_done = tnext >= 0;
+        c->wrap_left_done = fuse != FUSE_NONE;
     }
     return 0;
 }
@@ -656,7 +665,7 @@ int do_sweep_up(dqmc_ctx* c)
         const bool push = t % c->pace == 0 || t == L;
         RCHECK(wrap_up(c, t % L));
         // unless a stabilization follows, the next step is wrap_up(t + 1)
-        RCHECK(metropolis(c, t % L, (fuse && !push) ? (t + 1) % L : -1));
+        RCHECK(metropolis(c, t % L, (fuse && !push) ? FUSE_UP : FUSE_NONE, (t + 1) % L));
         RCHECK(chain_mul(c, t % L, false, push));
         if (push) {
             c->right.len -= 1;
@@ -673,10 +682,13 @@ int do_sweep_down(dqmc_ctx* c)
 {
     const int L = c->L;
     const bool store = !c->thermalization && c->eqtime;
+    const bool fuse = wrap_fusable(c);
+    c->wrap_left_done = false;
     chain_begin(c, c->right);
     for (int t = L; t >= 1; --t) {
         const bool push = (t - 1) % c->pace == 0;
-        RCHECK(metropolis(c, t % L));
+        // the next step on G is always wrap_down(t)
+        RCHECK(metropolis(c, t % L, fuse ? FUSE_DOWN : FUSE_NONE));
         RCHECK(chain_mul(c, t % L, true, push));
         RCHECK(wrap_down(c, t % L));
         if (push) {
@@ -695,6 +707,7 @@ int do_sweep_dynamic(dqmc_ctx* c)
     const int L = c->L, N = c->N;
     const long long nn = c->NN(), sv = (long long)L * nn;
     c->gt0 = c->gt0A; c->g0t = c->g0tA;
+    c->wrap_left_done = false;
     if (c->prec == DQMC_MIXED) HCHECK(c, launch_copy_f2d(c->G32, nn, c->W1, nn, nn, c->B, c->stream));
     else HCHECK(c, launch_copy_d(c->G64, nn, c->W1, nn, nn, c->B, c->stream));
     HCHECK(c, launch_copy_d(c->W1, nn, c->gt0, nn, nn, c->B, c->stream));

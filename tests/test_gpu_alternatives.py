@@ -21,7 +21,9 @@ one exercised).
   DQMC_LOOKAHEAD=0       blocked QR and LU panels and updates in order on one stream, instead of
                          each next panel beside the rest of the previous update on a second stream
   DQMC_WRAP_FUSE=0       every wrap's left product by its own launch, instead of by the Metropolis
-                         workgroups of the slice before (up sweeps without a stabilization between)
+                         workgroups of the slice (up sweeps: the wrap to the next slice unless a
+                         stabilization comes between; down sweeps: the slice's own wrap, its left
+                         product before its right one)
   DQMC_CHAIN=slice       each B-chain factor applied when its slice is done, instead of all factors
                          of a stack block in one pass at its push (also checked at pace 17, where
                          a block takes two passes of at most 16 factors)
