@@ -326,17 +326,17 @@ def main():
     avg_s = ms / n / 1e3
     # Algorithmic work of one launch (one slice of every chain): one read and one write of each
     # chain's N x N FP64 G, and the accepted flips' rank-1 Sherman-Morrison updates, 2 N^2 flops
-    # each (src/hubbard.cpp:132-139).  The launches also form a wrap's left product, exp(-+dt K) G,
-    # in an up sweep for the slices not followed by a stabilization (NT - S of the NT) and in a down
-    # sweep for every slice: one more read of G and one write of an N x N result, and 2 (16^3 + 16^3)
-    # flops per column with the Kronecker factors.  The two timed sweeps are one up and one down
-    # sweep.  The kernel moves several times the algorithmic bytes (the flushes of its site blocks),
-    # so HBM is what bounds it.
+    # each (src/hubbard.cpp:132-139).  The launches also apply the slice wrap that follows, in an up
+    # sweep for the slices not followed by a stabilization (NT - S of the NT) and in a down sweep for
+    # every slice: its left product (G read, the N x N product written) and its right product (that
+    # product read, G written), each 2 (16^3 + 16^3) flops per lattice vector with the Kronecker
+    # factors.  The two timed sweeps are one up and one down sweep.  The kernel moves several times
+    # the algorithmic bytes (the flushes of its site blocks), so HBM is what bounds it.
     S = -(-NT // PACE)
     fused = (2 * NT - S) if os.environ.get("DQMC_WRAP_FUSE", "") != "0" else 0
     ffrac = fused / float(n)
-    abytes = 2.0 * N * N * 8 * args.chains * (1.0 + ffrac)
-    flops = 2.0 * N * N * accs / n + ffrac * 4.0 * 16 ** 3 * N * args.chains
+    abytes = 2.0 * N * N * 8 * args.chains * (1.0 + 2.0 * ffrac)
+    flops = 2.0 * N * N * accs / n + ffrac * 2.0 * 4.0 * 16 ** 3 * N * args.chains
     roof = {"bound": "hbm", "achieved": round(abytes / avg_s / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = round(roof["achieved"] / roof["peak"], 5)
     roof["traffic"] = None
@@ -345,7 +345,7 @@ def main():
     roof["avg_launch_us"] = round(avg_s * 1e6, 2)
     roof["algorithmic_bytes_per_launch"] = round(abytes)
     roof["flops_per_launch"] = round(flops)
-    roof["launches_with_left_product"] = "%d of %d" % (fused, n)
+    roof["launches_with_wrap"] = "%d of %d" % (fused, n)
     roof["achieved_tflops"] = round(flops / avg_s / 1e12, 3)
     if roof.get("traffic"):
         roof["traffic_gbs"] = round(roof["traffic"] / avg_s / 1e9, 1)

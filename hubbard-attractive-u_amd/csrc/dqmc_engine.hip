@@ -127,10 +127,11 @@ struct dqmc_ctx {
     int chain_pend[KCHAIN_MAX] = {};
     int chain_npend = 0;
     bool chain_pend_adj = false;
-    // FP64 16 x 16 up sweeps: the Metropolis workgroup of slice t also applies the left product of the
-    // wrap to slice t + 1 (into W1) when no stabilization comes between (DQMC_WRAP_FUSE=0 turns it off)
+    // FP64 16 x 16 sweeps: the Metropolis workgroups of a slice also apply the wrap that follows it
+    // (up sweeps: to the next slice unless a stabilization comes between; down sweeps: the slice's
+    // own); DQMC_WRAP_FUSE=0 turns it off
     bool wrap_fuse = true;
-    bool wrap_left_done = false;
+    bool wrap_done = false;
     double *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *qr = nullptr;
     int* perm = nullptr;
     double *gt0A = nullptr, *gt0B = nullptr, *g0tA = nullptr, *g0tB = nullptr, *gt0 = nullptr, *g0t = nullptr;
@@ -352,9 +353,12 @@ int wrap_up(dqmc_ctx* c, int t)
             RCHECK(kron<float>(c, true, c->G32, c->T32, c->E1f, dfield(c, c->tab, t)));
             RCHECK(kron<float>(c, false, c->T32, c->G32, c->F1f, dnone(), dnone(), dfield(c, c->itab, t)));
         } else {
-            // W1 = D (E(x)E) G is already there when the previous slice's Metropolis workgroups made it
-            if (!c->wrap_left_done) RCHECK(kron<double>(c, true, c->G64, c->W1, c->E1, dfield(c, c->tab, t)));
-            c->wrap_left_done = false;
+            // the previous slice's Metropolis workgroups may have applied this wrap already
+            if (c->wrap_done) {
+                c->wrap_done = false;
+                return 0;
+            }
+            RCHECK(kron<double>(c, true, c->G64, c->W1, c->E1, dfield(c, c->tab, t)));
             RCHECK(kron<double>(c, false, c->W1, c->G64, c->F1, dnone(), dnone(), dfield(c, c->itab, t)));
         }
         return 0;
@@ -380,12 +384,10 @@ int wrap_down(dqmc_ctx* c, int t)
         if (c->prec == DQMC_MIXED) {
             RCHECK(kron<float>(c, false, c->G32, c->T32, c->E1f, dnone(), dfield(c, c->tab, t)));
             RCHECK(kron<float>(c, true, c->T32, c->G32, c->F1f, dnone(), dfield(c, c->itab, t)));
-        } else if (c->wrap_left_done) {
-            // W1 = (F(x)F) D^{-1} G was formed by the slice's Metropolis workgroups: the right product
-            // follows the left one here (the reference applies B from the right first; the two orders
-            // agree to rounding)
-            c->wrap_left_done = false;
-            RCHECK(kron<double>(c, false, c->W1, c->G64, c->E1, dnone(), dfield(c, c->tab, t)));
+        } else if (c->wrap_done) {
+            // applied by the slice's Metropolis workgroups, left product first (the reference applies
+            // B from the right first; the two orders agree to rounding)
+            c->wrap_done = false;
         } else {
             RCHECK(kron<double>(c, false, c->G64, c->W1, c->E1, dnone(), dfield(c, c->tab, t)));
             RCHECK(kron<double>(c, true, c->W1, c->G64, c->F1, dnone(), dfield(c, c->itab, t)));
@@ -427,9 +429,10 @@ bool wrap_fusable(dqmc_ctx* c)
 {
     return c->wrap_fuse && c->kron && c->prec == DQMC_FP64 && c->nl == 16;
 }
-// Metropolis update of slice t.  With a fused wrap the workgroups then form the left product of the next
-// wrap into W1, and that wrap applies only its right product: FUSE_UP forms D(t') (E(x)E) G for
-// wrap_up(t') with t' = lt, FUSE_DOWN forms (F(x)F) D(t)^{-1} G for wrap_down(t).
+// Metropolis update of slice t.  With a fused wrap the workgroups then apply the next wrap as well
+// (the left product into W1, the right one back into G), and that wrap call does nothing: FUSE_UP is
+// wrap_up(t') with t' = lt, G <- D(t') (E(x)E) G (F(x)F) D(t')^{-1}; FUSE_DOWN is wrap_down(t),
+// G <- ((F(x)F) D(t)^{-1} G) D(t) (E(x)E).
 enum WrapFuse { FUSE_NONE = 0, FUSE_UP = 1, FUSE_DOWN = 2 };
 int metropolis(dqmc_ctx* c, int t, int fuse = FUSE_NONE, int lt = 0)
 {
@@ -448,11 +451,13 @@ int metropolis(dqmc_ctx* c, int t, int fuse = FUSE_NONE, int lt = 0)
         a.accmask = c->accmask; a.acc_count = c->acc_count; a.health = c->health; a.kmax = c->kmax;
         if (fuse == FUSE_UP) {
             a.lout = c->W1; a.lE = c->E1; a.ltab = c->tab; a.lt = lt; a.lmode = 1;
+            a.rE = c->F1; a.rtab = c->itab; a.rmode = 1;
         } else if (fuse == FUSE_DOWN) {
             a.lout = c->W1; a.lE = c->F1; a.ltab = c->itab; a.lt = t; a.lmode = 2;
+            a.rE = c->E1; a.rtab = c->tab; a.rmode = 2;
         }
         TIMED(c, KT_METRO, launch_metropolis<double>(a, c->B, c->metro_threads, c->stream));
-        c->wrap_left_done = fuse != FUSE_NONE;
+        c->wrap_done = fuse != FUSE_NONE;
     }
     return 0;
 }
@@ -659,7 +664,7 @@ int do_sweep_up(dqmc_ctx* c)
     const int L = c->L;
     const bool store = !c->thermalization && c->eqtime;
     const bool fuse = wrap_fusable(c);
-    c->wrap_left_done = false;
+    c->wrap_done = false;
     chain_begin(c, c->left);
     for (int t = 1; t <= L; ++t) {
         const bool push = t % c->pace == 0 || t == L;
@@ -683,7 +688,7 @@ int do_sweep_down(dqmc_ctx* c)
     const int L = c->L;
     const bool store = !c->thermalization && c->eqtime;
     const bool fuse = wrap_fusable(c);
-    c->wrap_left_done = false;
+    c->wrap_done = false;
     chain_begin(c, c->right);
     for (int t = L; t >= 1; --t) {
         const bool push = (t - 1) % c->pace == 0;
@@ -707,7 +712,7 @@ int do_sweep_dynamic(dqmc_ctx* c)
     const int L = c->L, N = c->N;
     const long long nn = c->NN(), sv = (long long)L * nn;
     c->gt0 = c->gt0A; c->g0t = c->g0tA;
-    c->wrap_left_done = false;
+    c->wrap_done = false;
     if (c->prec == DQMC_MIXED) HCHECK(c, launch_copy_f2d(c->G32, nn, c->W1, nn, nn, c->B, c->stream));
     else HCHECK(c, launch_copy_d(c->G64, nn, c->W1, nn, nn, c->B, c->stream));
     HCHECK(c, launch_copy_d(c->W1, nn, c->gt0, nn, nn, c->B, c->stream));

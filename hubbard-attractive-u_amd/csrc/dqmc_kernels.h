@@ -126,13 +126,18 @@ struct MetroArgs {
     unsigned long long* acc_count;    // [chain]
     uint32_t* health;                 // [chain] HEALTH_* flags
     int kmax;
-    // Optional left product of the next slice wrap, done by each chain's workgroup after its slice
-    // (FP64, N = 256): lout = diag(r) (E(x)E) diag(s) G with the diagonal of slice lt from ltab as r
-    // (lmode 1) or s (lmode 2); lmode 0 skips it.
+    // Optional slice wrap, done by each chain's workgroup after its slice (FP64, N = 256):
+    //   left product  lout = diag(r) (E(x)E) diag(s) G with the diagonal of slice lt from ltab as r
+    //                 (lmode 1) or s (lmode 2); lmode 0 skips the wrap;
+    //   right product G = lout diag(s') (F(x)F) diag(c') with the diagonal of slice lt from rtab as c'
+    //                 (rmode 1) or s' (rmode 2), factor rE; rmode 0 leaves lout as the result.
     double* lout = nullptr;
     const double* lE = nullptr;
     const double* ltab = nullptr;
     int lt = 0, lmode = 0;
+    const double* rE = nullptr;
+    const double* rtab = nullptr;
+    int rmode = 0;
 };
 // threads: workgroup size for N >= 256 (256, 512 or 1024); smaller lattices use 2 x N rounded
 // up to a wave.

@@ -400,6 +400,64 @@ __device__ __forceinline__ void metro_left16(const MetroArgs<double>& a, int b, 
     }
 }
 
+// The wrap's right product by the workgroup of chain b after its left product (FP64, N = 256): G =
+// lout diag(s') (F(x)F) diag(c'), 64 rows of lout at a time staged row-major in LDS (row stride 273,
+// one pad word per 16 sites, as kron_right16_kernel does for 32 rows), each row transformed by one
+// wave; the same per-element arithmetic as kron_right16_kernel.
+#define MR_ROWS 64
+#define MR_LD 273
+__device__ __forceinline__ int mr_pad(int site) { return site + (site >> 4); }
+template <int MT>
+__device__ __forceinline__ void metro_right16(const MetroArgs<double>& a, int b, unsigned char* smem)
+{
+    double* tile = reinterpret_cast<double*>(smem);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NWV = MT / 64;
+    const double* X = a.lout + (long long)b * a.sG;
+    double* C = a.G + (long long)b * a.sG;
+    const int8_t* f = a.fields + ((long long)b * a.L + a.lt) * 256;
+    const double dp = a.rtab[2 * b], dm = a.rtab[2 * b + 1];
+    const bool inner = a.rmode == 2;
+    double ef[4], ef2[4];
+    load_efrag<double>(a.rE, lane, ef, ef2);
+    for (int r0 = 0; r0 < 256; r0 += MR_ROWS) {
+        for (int e = tid; e < MR_ROWS / 2 * 256; e += MT) {
+            const int rr = 2 * (e & (MR_ROWS / 2 - 1)), c = e / (MR_ROWS / 2);
+            const double2 v = *reinterpret_cast<const double2*>(X + r0 + rr + (long long)c * 256);
+            double v0 = v.x, v1 = v.y;
+            if (inner) {
+                const double d = f[c] > 0 ? dp : dm;
+                v0 = v0 * d; v1 = v1 * d;
+            }
+            tile[rr * MR_LD + mr_pad(c)] = v0;
+            tile[(rr + 1) * MR_LD + mr_pad(c)] = v1;
+        }
+        __syncthreads();
+        for (int q = wave; q < MR_ROWS; q += NWV) {
+            double* row = tile + q * MR_LD;
+            double vin[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) vin[s] = row[mr_pad(lane + 64 * s)];
+            d4_t z;
+            kron16_vec<double>(ef, ef2, vin, z);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) row[mr_pad(Mfma<double>::row(lane, r) + 16 * (lane & 15))] = z[r];
+        }
+        __syncthreads();
+        for (int e = tid; e < MR_ROWS / 2 * 256; e += MT) {
+            const int rr = 2 * (e & (MR_ROWS / 2 - 1)), c = e / (MR_ROWS / 2);
+            double v0 = tile[rr * MR_LD + mr_pad(c)], v1 = tile[(rr + 1) * MR_LD + mr_pad(c)];
+            if (!inner) {
+                const double d = f[c] > 0 ? dp : dm;
+                v0 *= d; v1 *= d;
+            }
+            double2 v; v.x = v0; v.y = v1;
+            *reinterpret_cast<double2*>(C + r0 + rr + (long long)c * 256) = v;
+        }
+        __syncthreads();   // the next rows overwrite the tile
+    }
+}
+
 // Submatrix Metropolis update of slice t.  Sites are visited in blocks S of kb consecutive
 // sites.  Inside a block the accept/reject sequence runs on Gs = G(S,S) in LDS with the
 // reference's rank-1 update.  After the block the accepted flips A are applied to the whole
@@ -517,7 +575,13 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
     }
     if constexpr (sizeof(T) == 8) {
         // the final flush's stores of G are visible to the whole workgroup past the barrier above
-        if (a.lmode != 0) metro_left16<MT>(a, b, smem);
+        if (a.lmode != 0) {
+            metro_left16<MT>(a, b, smem);
+            if (a.rmode != 0) {
+                __syncthreads();   // lout complete, and the left product's transpose buffers free
+                metro_right16<MT>(a, b, smem);
+            }
+        }
     }
 }
 
@@ -543,6 +607,10 @@ hipError_t launch_metropolis(const MetroArgs<T>& a, int batch, int threads, hipS
         return hipErrorInvalidValue;
     size_t shm = metropolis_lds_bytes(a.N, a.kmax, sizeof(T));
     if (a.lmode != 0 && shm < (size_t)16 * 16 * 17 * 8) shm = (size_t)16 * 16 * 17 * 8;   // the epilogue's transpose buffers
+    if (a.rmode != 0) {
+        if (a.lmode == 0 || !a.rE || !a.rtab) return hipErrorInvalidValue;
+        if (shm < (size_t)MR_ROWS * MR_LD * 8) shm = (size_t)MR_ROWS * MR_LD * 8;   // the right product's row tile
+    }
     if (shm > 160 * 1024) return hipErrorInvalidValue;
     if (a.N < 256) threads = ((a.N + 63) / 64) * 64 * 2 <= 1024 ? ((a.N + 63) / 64) * 64 * 2 : 1024;
     if (threads <= 256)
