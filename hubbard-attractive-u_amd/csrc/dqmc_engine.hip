@@ -669,8 +669,9 @@ int do_sweep_up(dqmc_ctx* c)
     for (int t = 1; t <= L; ++t) {
         const bool push = t % c->pace == 0 || t == L;
         RCHECK(wrap_up(c, t % L));
-        // unless a stabilization follows, the next step is wrap_up(t + 1)
-        RCHECK(metropolis(c, t % L, (fuse && !push) ? FUSE_UP : FUSE_NONE, (t + 1) % L));
+        // unless a stabilization follows, the next step is wrap_up(t + 1); when measuring, G(t) is
+        // stored first
+        RCHECK(metropolis(c, t % L, (fuse && !push && !store) ? FUSE_UP : FUSE_NONE, (t + 1) % L));
         RCHECK(chain_mul(c, t % L, false, push));
         if (push) {
             c->right.len -= 1;

@@ -98,3 +98,38 @@ def compare_runs(alt, wrap, ref, ref_wrap, wrap_max=1e-8, tol=TOL_G):
             assert d < tol, "chain %d sweep %d: max|dG| %g" % (c, sweep, d)
         say("  sweep %d: decisions identical, max|dG| = %.2e" %
             (sweep, max(np.abs(ga[c] - gr[c]).max() for c in range(NCH))))
+
+
+def run_stored(n_sweeps):
+    """Measuring sweeps (equal-time G of every slice stored) on the 16x16 workload."""
+    e = gpu.Engine(NL, NT, BETA, U, mu=MU, pace=PACE, n_chains=NCH, delay_k=K)
+    for c in range(NCH):
+        e.set_seed(c, 4300 + c)
+        e.fields_random(c)
+    e.set_collect(True, False)
+    e.init()
+    e.set_thermalization(False)
+    out = []
+    for sweep in range(n_sweeps):
+        e.sweep_up() if sweep % 2 == 0 else e.sweep_down()
+        out.append(([e.accept_mask(c).copy() for c in range(NCH)],
+                    [np.stack([e.gf(c, gpu.GF_TT, t) for t in range(NT)]) for c in range(NCH)]))
+    e.close()
+    return out
+
+
+def test_stored_gf_with_fused_wraps(monkeypatch):
+    # The slices' wraps run inside the Metropolis launches; the G(t) stored for measurements must
+    # still be the one of each slice after its update (up sweeps) or after its wrap (down sweeps), as
+    # with separate wraps.
+    say("case: stored G(t), fused wraps against DQMC_WRAP_FUSE=0, 16x16 L=40, 8 chains")
+    fused = run_stored(2)
+    monkeypatch.setenv("DQMC_WRAP_FUSE", "0")
+    plain = run_stored(2)
+    for sweep, ((ma, ga), (mr, gr)) in enumerate(zip(fused, plain)):
+        for c in range(NCH):
+            assert np.array_equal(ma[c], mr[c]), "chain %d sweep %d: accept decisions differ" % (c, sweep)
+            d = np.abs(ga[c] - gr[c]).max()
+            assert d < TOL_G, "chain %d sweep %d: max|dG(t)| %g" % (c, sweep, d)
+        say("  sweep %d: decisions identical, max|dG(t)| = %.2e" %
+            (sweep, max(np.abs(ga[c] - gr[c]).max() for c in range(NCH))))
