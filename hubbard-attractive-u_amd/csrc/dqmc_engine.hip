@@ -132,6 +132,11 @@ struct dqmc_ctx {
     // own); DQMC_WRAP_FUSE=0 turns it off
     bool wrap_fuse = true;
     bool wrap_done = false;
+    // Metropolis launches of the two halves of the chains on the two streams between the points where
+    // all chains meet again (metro_join); DQMC_METRO_SPLIT=0 launches all chains at once
+    bool metro_split = true;
+    bool split_open = false;
+    hipEvent_t ms_fork = nullptr, ms_join = nullptr;
     double *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *qr = nullptr;
     int* perm = nullptr;
     double *gt0A = nullptr, *gt0B = nullptr, *g0tA = nullptr, *g0tB = nullptr, *gt0 = nullptr, *g0t = nullptr;
@@ -342,9 +347,19 @@ int kron(dqmc_ctx* c, bool left, const T* X, T* C, const T* E, DiagArg rowd = Di
 }
 
 // ---- slice operations --------------------------------------------------------------------
+// All chains meet again: the main stream waits for the second stream's Metropolis launches.
+int metro_join(dqmc_ctx* c)
+{
+    if (!c->split_open) return 0;
+    HCHECK(c, hipEventRecord(c->ms_join, c->stream2));
+    HCHECK(c, hipStreamWaitEvent(c->stream, c->ms_join, 0));
+    c->split_open = false;
+    return 0;
+}
 // G <- B(t) G B(t)^{-1}  (Core::wrap_from_0_to_beta, src/dqmc_core.cpp:142-147)
 int wrap_up(dqmc_ctx* c, int t)
 {
+    if (!c->wrap_done) RCHECK(metro_join(c));
     const long long nn = c->NN();
     const int N = c->N;
     if (c->kron) {
@@ -377,6 +392,7 @@ int wrap_up(dqmc_ctx* c, int t)
 // G <- B(t)^{-1} G B(t)  (Core::wrap_from_beta_to_0, src/dqmc_core.cpp:156-161)
 int wrap_down(dqmc_ctx* c, int t)
 {
+    if (!c->wrap_done) RCHECK(metro_join(c));
     const long long nn = c->NN();
     const int N = c->N;
     if (c->kron) {
@@ -456,7 +472,23 @@ int metropolis(dqmc_ctx* c, int t, int fuse = FUSE_NONE, int lt = 0)
             a.lout = c->W1; a.lE = c->F1; a.ltab = c->itab; a.lt = t; a.lmode = 2;
             a.rE = c->E1; a.rtab = c->tab; a.rmode = 2;
         }
-        TIMED(c, KT_METRO, launch_metropolis<double>(a, c->B, c->metro_threads, c->stream));
+        if (c->metro_split && c->B >= 16 && c->kt_kernel < 0) {
+            // The halves run on the two streams, each in slice order; the chains are independent, so a
+            // half starts its next slice as soon as its own slowest chain is done, and fills the CUs
+            // the other half's tail leaves idle.
+            if (!c->split_open) {
+                HCHECK(c, hipEventRecord(c->ms_fork, c->stream));
+                HCHECK(c, hipStreamWaitEvent(c->stream2, c->ms_fork, 0));
+                c->split_open = true;
+            }
+            const int h = c->B / 2;
+            a.chain0 = 0;
+            HCHECK(c, launch_metropolis<double>(a, h, c->metro_threads, c->stream));
+            a.chain0 = h;
+            HCHECK(c, launch_metropolis<double>(a, c->B - h, c->metro_threads, c->stream2));
+        } else {
+            TIMED(c, KT_METRO, launch_metropolis<double>(a, c->B, c->metro_threads, c->stream));
+        }
         c->wrap_done = fuse != FUSE_NONE;
     }
     return 0;
@@ -497,6 +529,7 @@ const double* chain_operand(dqmc_ctx* c, long long& sX)
 int chain_flush(dqmc_ctx* c, bool last)
 {
     if (c->chain_npend == 0) return 0;
+    RCHECK(metro_join(c));   // the factors read the slices' fields
     DStack& s = *c->chain_stack;
     KronChainArgs k;
     k.X = chain_operand(c, k.sX);
@@ -527,6 +560,7 @@ int chain_mul(dqmc_ctx* c, int t, bool adjoint, bool last)
         if (last || c->chain_npend == KCHAIN_MAX) RCHECK(chain_flush(c, last));
         return 0;
     }
+    RCHECK(metro_join(c));   // the factor reads the slice's fields
     long long sX = nn;
     const double* X = chain_operand(c, sX);
     const DiagArg d = dfield(c, c->tab, t);
@@ -642,6 +676,7 @@ int recompute_dyn(dqmc_ctx* c, double* gtt, double* gt0n, double* g0tn)
 
 int store_current(dqmc_ctx* c, double* vec, int t)
 {
+    RCHECK(metro_join(c));
     const long long nn = c->NN(), sv = (long long)c->L * nn;
     if (c->prec == DQMC_MIXED)
         HCHECK(c, launch_copy_f2d(c->G32, nn, vec + (long long)t * nn, sv, nn, c->B, c->stream));
@@ -652,6 +687,7 @@ int store_current(dqmc_ctx* c, double* vec, int t)
 
 int stabilize_eq(dqmc_ctx* c)
 {
+    RCHECK(metro_join(c));
     RCHECK(recompute_eq(c, c->W3));
     HCHECK(c, launch_compare_commit(c->W3, c->G64, c->prec == DQMC_MIXED ? c->G32 : nullptr, c->NN(), c->N, c->wrap_eq,
                                     c->maxdG, c->wd_bound, c->health, c->cmp_part, c->B, c->stream));
@@ -681,7 +717,7 @@ int do_sweep_up(dqmc_ctx* c)
         }
         if (store) RCHECK(store_current(c, c->vecgftt, t % L));
     }
-    return 0;
+    return metro_join(c);
 }
 
 int do_sweep_down(dqmc_ctx* c)
@@ -705,7 +741,7 @@ int do_sweep_down(dqmc_ctx* c)
         }
         if (store) RCHECK(store_current(c, c->vecgftt, (t - 1) % L));
     }
-    return 0;
+    return metro_join(c);
 }
 
 int do_sweep_dynamic(dqmc_ctx* c)
@@ -989,6 +1025,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         c->trsm_v1 = tr && std::string(tr) == "v1";
         const char* w = std::getenv("DQMC_WRAP");
         c->kron = !(w && std::string(w) == "dense");
+        const char* msp = std::getenv("DQMC_METRO_SPLIT");
+        c->metro_split = !(msp && std::string(msp) == "0");
         const char* wf = std::getenv("DQMC_WRAP_FUSE");
         c->wrap_fuse = !(wf && std::string(wf) == "0");
         const char* ch = std::getenv("DQMC_CHAIN");
@@ -1017,6 +1055,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess) return c->fail(DQMC_EHIP, "stream");
     HCHECK(c, hipEventCreateWithFlags(&c->la_fork, hipEventDisableTiming));
     HCHECK(c, hipEventCreateWithFlags(&c->la_join, hipEventDisableTiming));
+    HCHECK(c, hipEventCreateWithFlags(&c->ms_fork, hipEventDisableTiming));
+    HCHECK(c, hipEventCreateWithFlags(&c->ms_join, hipEventDisableTiming));
     HCHECK(c, prepare_kernels());
     for (auto& e : c->ev) HCHECK(c, hipEventCreate(&e));
     RCHECK(c->alloc(&c->expK, nn * 8)); RCHECK(c->alloc(&c->invExpK, nn * 8));
@@ -1120,6 +1160,8 @@ void dqmc_destroy(dqmc_ctx* c)
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->la_fork) (void)hipEventDestroy(c->la_fork);
     if (c->la_join) (void)hipEventDestroy(c->la_join);
+    if (c->ms_fork) (void)hipEventDestroy(c->ms_fork);
+    if (c->ms_join) (void)hipEventDestroy(c->ms_join);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

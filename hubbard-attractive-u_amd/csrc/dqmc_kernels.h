@@ -126,6 +126,7 @@ struct MetroArgs {
     unsigned long long* acc_count;    // [chain]
     uint32_t* health;                 // [chain] HEALTH_* flags
     int kmax;
+    int chain0 = 0;                   // workgroup w runs chain chain0 + w
     // Optional slice wrap, done by each chain's workgroup after its slice (FP64, N = 256):
     //   left product  lout = diag(r) (E(x)E) diag(s) G with the diagonal of slice lt from ltab as r
     //                 (lmode 1) or s (lmode 2); lmode 0 skips the wrap;

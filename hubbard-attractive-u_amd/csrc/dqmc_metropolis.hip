@@ -473,7 +473,7 @@ __global__ __launch_bounds__(MT) __attribute__((amdgpu_waves_per_eu(MT / 256, MT
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const int N = a.N, kb = a.kmax;
-    const int b = blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
+    const int b = a.chain0 + blockIdx.x, tid = threadIdx.x, nth = blockDim.x;
     MetroLds<T> L = metro_lds<T>(smem, N, kb);
 
     T* G = a.G + (long long)b * a.sG;
