@@ -55,6 +55,7 @@ double host_canonical(uint64_t& x)
 
 }  // namespace
 
+#define MSPLIT_MAX 4
 struct dqmc_ctx {
     // parameters
     int nl = 0, N = 0, L = 0, pace = 0, S = 0, B = 0, prec = DQMC_FP64, kmax = 1;
@@ -132,11 +133,12 @@ struct dqmc_ctx {
     // own); DQMC_WRAP_FUSE=0 turns it off
     bool wrap_fuse = true;
     bool wrap_done = false;
-    // Metropolis launches of the two halves of the chains on the two streams between the points where
-    // all chains meet again (metro_join); DQMC_METRO_SPLIT=0 launches all chains at once
-    bool metro_split = true;
+    // Metropolis launches of groups of the chains on their own streams between the points where all
+    // chains meet again (metro_join); DQMC_METRO_SPLIT=n sets the group count (1: all chains at once)
+    int metro_groups = 2;
     bool split_open = false;
-    hipEvent_t ms_fork = nullptr, ms_join = nullptr;
+    hipStream_t ms_stream[MSPLIT_MAX] = {};   // [0] is the main stream, [1] the second one
+    hipEvent_t ms_fork = nullptr, ms_join[MSPLIT_MAX] = {};
     double *W1 = nullptr, *W2 = nullptr, *W3 = nullptr, *qr = nullptr;
     int* perm = nullptr;
     double *gt0A = nullptr, *gt0B = nullptr, *g0tA = nullptr, *g0tB = nullptr, *gt0 = nullptr, *g0t = nullptr;
@@ -351,8 +353,10 @@ int kron(dqmc_ctx* c, bool left, const T* X, T* C, const T* E, DiagArg rowd = Di
 int metro_join(dqmc_ctx* c)
 {
     if (!c->split_open) return 0;
-    HCHECK(c, hipEventRecord(c->ms_join, c->stream2));
-    HCHECK(c, hipStreamWaitEvent(c->stream, c->ms_join, 0));
+    for (int g = 1; g < c->metro_groups; ++g) {
+        HCHECK(c, hipEventRecord(c->ms_join[g], c->ms_stream[g]));
+        HCHECK(c, hipStreamWaitEvent(c->stream, c->ms_join[g], 0));
+    }
     c->split_open = false;
     return 0;
 }
@@ -472,20 +476,21 @@ int metropolis(dqmc_ctx* c, int t, int fuse = FUSE_NONE, int lt = 0)
             a.lout = c->W1; a.lE = c->F1; a.ltab = c->itab; a.lt = t; a.lmode = 2;
             a.rE = c->E1; a.rtab = c->tab; a.rmode = 2;
         }
-        if (c->metro_split && c->B >= 16 && c->kt_kernel < 0) {
-            // The halves run on the two streams, each in slice order; the chains are independent, so a
-            // half starts its next slice as soon as its own slowest chain is done, and fills the CUs
-            // the other half's tail leaves idle.
+        const int ng = c->metro_groups;
+        if (ng > 1 && c->B >= 8 * ng && c->kt_kernel < 0) {
+            // The groups run on their own streams, each in slice order; the chains are independent, so
+            // a group starts its next slice as soon as its own slowest chain is done, and fills the CUs
+            // the other groups' tails leave idle.
             if (!c->split_open) {
                 HCHECK(c, hipEventRecord(c->ms_fork, c->stream));
-                HCHECK(c, hipStreamWaitEvent(c->stream2, c->ms_fork, 0));
+                for (int g = 1; g < ng; ++g) HCHECK(c, hipStreamWaitEvent(c->ms_stream[g], c->ms_fork, 0));
                 c->split_open = true;
             }
-            const int h = c->B / 2;
-            a.chain0 = 0;
-            HCHECK(c, launch_metropolis<double>(a, h, c->metro_threads, c->stream));
-            a.chain0 = h;
-            HCHECK(c, launch_metropolis<double>(a, c->B - h, c->metro_threads, c->stream2));
+            for (int g = 0; g < ng; ++g) {
+                const int b0 = (int)((long long)c->B * g / ng), b1 = (int)((long long)c->B * (g + 1) / ng);
+                a.chain0 = b0;
+                HCHECK(c, launch_metropolis<double>(a, b1 - b0, c->metro_threads, c->ms_stream[g]));
+            }
         } else {
             TIMED(c, KT_METRO, launch_metropolis<double>(a, c->B, c->metro_threads, c->stream));
         }
@@ -1026,7 +1031,14 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         const char* w = std::getenv("DQMC_WRAP");
         c->kron = !(w && std::string(w) == "dense");
         const char* msp = std::getenv("DQMC_METRO_SPLIT");
-        c->metro_split = !(msp && std::string(msp) == "0");
+        if (msp) {
+            const int v = std::atoi(msp);
+            if (v != 0 && v != 1 && v != 2 && v != 4) {
+                delete c;
+                return free_error(DQMC_EINVAL, "dqmc_create: DQMC_METRO_SPLIT must be 1, 2 or 4");
+            }
+            c->metro_groups = v < 1 ? 1 : v;
+        }
         const char* wf = std::getenv("DQMC_WRAP_FUSE");
         c->wrap_fuse = !(wf && std::string(wf) == "0");
         const char* ch = std::getenv("DQMC_CHAIN");
@@ -1056,7 +1068,11 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     HCHECK(c, hipEventCreateWithFlags(&c->la_fork, hipEventDisableTiming));
     HCHECK(c, hipEventCreateWithFlags(&c->la_join, hipEventDisableTiming));
     HCHECK(c, hipEventCreateWithFlags(&c->ms_fork, hipEventDisableTiming));
-    HCHECK(c, hipEventCreateWithFlags(&c->ms_join, hipEventDisableTiming));
+    c->ms_stream[0] = c->stream;
+    c->ms_stream[1] = c->stream2;
+    for (int g = 2; g < c->metro_groups; ++g)
+        if (hipStreamCreateWithFlags(&c->ms_stream[g], hipStreamNonBlocking) != hipSuccess) return c->fail(DQMC_EHIP, "stream");
+    for (int g = 1; g < c->metro_groups; ++g) HCHECK(c, hipEventCreateWithFlags(&c->ms_join[g], hipEventDisableTiming));
     HCHECK(c, prepare_kernels());
     for (auto& e : c->ev) HCHECK(c, hipEventCreate(&e));
     RCHECK(c->alloc(&c->expK, nn * 8)); RCHECK(c->alloc(&c->invExpK, nn * 8));
@@ -1161,7 +1177,10 @@ void dqmc_destroy(dqmc_ctx* c)
     if (c->la_fork) (void)hipEventDestroy(c->la_fork);
     if (c->la_join) (void)hipEventDestroy(c->la_join);
     if (c->ms_fork) (void)hipEventDestroy(c->ms_fork);
-    if (c->ms_join) (void)hipEventDestroy(c->ms_join);
+    for (int g = 1; g < MSPLIT_MAX; ++g)
+        if (c->ms_join[g]) (void)hipEventDestroy(c->ms_join[g]);
+    for (int g = 2; g < MSPLIT_MAX; ++g)
+        if (c->ms_stream[g]) (void)hipStreamDestroy(c->ms_stream[g]);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

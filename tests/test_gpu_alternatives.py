@@ -24,8 +24,9 @@ one exercised).
                          workgroups of the slice (up sweeps: the wrap to the next slice unless a
                          stabilization comes between; down sweeps: the slice's own wrap, its left
                          product before its right one)
-  DQMC_METRO_SPLIT=0     every Metropolis launch over all chains, instead of the two halves of the
-                         chains on two streams between the points where all chains meet again
+  DQMC_METRO_SPLIT=1|4   every Metropolis launch over all chains, or over four groups of the chains
+                         on four streams, instead of two halves on two streams between the points
+                         where all chains meet again
   DQMC_CHAIN=slice       each B-chain factor applied when its slice is done, instead of all factors
                          of a stack block in one pass at its push (also checked at pace 17, where
                          a block takes two passes of at most 16 factors)
@@ -72,7 +73,8 @@ def default_run():
 @pytest.mark.parametrize("var,val", [("DQMC_GEMM_TILE", "1"), ("DQMC_GEMM_TILE", "3"), ("DQMC_GEMM_TILE", "4"), ("DQMC_SOLVE", "qr"), ("DQMC_QR_PANEL", "1024"), ("DQMC_QR_UPDATE", "gemm"), ("DQMC_WRAP", "dense"), ("DQMC_TRSM", "v1"),
                                      ("DQMC_QR", "v1"), ("DQMC_METRO_THREADS", "256"),
                                      ("DQMC_METRO_THREADS", "512"), ("DQMC_LOOKAHEAD", "0"),
-                                     ("DQMC_CHAIN", "slice"), ("DQMC_WRAP_FUSE", "0"), ("DQMC_METRO_SPLIT", "0")])
+                                     ("DQMC_CHAIN", "slice"), ("DQMC_WRAP_FUSE", "0"), ("DQMC_METRO_SPLIT", "1"),
+                                     ("DQMC_METRO_SPLIT", "4")])
 def test_switch_follows_default_chains(default_run, monkeypatch, var, val):
     say("case: %s=%s, 16x16 L=40, 8 chains, 2 sweeps" % (var, val))
     monkeypatch.setenv(var, val)
