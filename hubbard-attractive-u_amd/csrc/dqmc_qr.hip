@@ -1186,14 +1186,15 @@ __global__ __launch_bounds__(256) void trsm_diag_kernel(double* __restrict__ buf
     double x[TRB];
 #pragma unroll
     for (int m = 0; m < TRB; ++m) x[m] = (m < nbk) ? Z[(long long)(k0 + m) * W] : 0.0;
+    // Right-looking substitution: once x_m is final, every row above it takes its term at once, so each
+    // step is one division and independent FMAs instead of a dependent chain of up to 31 FMAs.
 #pragma unroll
     for (int m = TRB - 1; m >= 0; --m) {
         if (m < nbk) {
-            double v = x[m];
+            const double xm = x[m] / Rk[m][m];
+            x[m] = xm;
 #pragma unroll
-            for (int q = m + 1; q < TRB; ++q)
-                if (q < nbk) v -= Rk[m][q] * x[q];
-            x[m] = v / Rk[m][m];
+            for (int p = 0; p < m; ++p) x[p] -= Rk[p][m] * xm;
         }
     }
 #pragma unroll
