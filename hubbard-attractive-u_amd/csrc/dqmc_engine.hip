@@ -87,6 +87,7 @@ struct dqmc_ctx {
         LookAhead a;
         if (!lookahead) return a;
         a.s2 = stream2; a.fork = la_fork; a.join = la_join; a.V2 = qrV2; a.T2 = qrT2; a.P2 = lu_perm2;
+        a.fuse = panel_fuse && !capturing;
         return a;
     }
     int current_t = 0;
@@ -133,6 +134,13 @@ struct dqmc_ctx {
     // own); DQMC_WRAP_FUSE=0 turns it off
     bool wrap_fuse = true;
     bool wrap_done = false;
+    // Eager sweeps: a look-ahead panel applies the previous update to its own columns (one launch and
+    // one kernel boundary fewer per panel; DQMC_PANEL_FUSE=0 turns it off).  In a captured graph both
+    // branches of a look-ahead step start together, and the combined launch, which then runs its update
+    // part beside the whole rest of the update, measured slower than a separate launch for those
+    // columns (3415 vs 3485 sweeps/s); eagerly the panel is dispatched first (3499 vs 3417).
+    bool panel_fuse = true;
+    bool capturing = false;
     // Metropolis launches of groups of the chains on their own streams between the points where all
     // chains meet again (metro_join); DQMC_METRO_SPLIT=n sets the group count (1: all chains at once)
     int metro_groups = 2;
@@ -916,7 +924,9 @@ int run_sweep(dqmc_ctx* c, sweep_fn fn, int kind)
     if (it == c->graphs.end()) {
         hipGraph_t g;
         HCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = true;
         int rc = fn(c);
+        c->capturing = false;
         hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (rc) return rc;
         HCHECK(c, e);
@@ -1030,6 +1040,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         c->trsm_v1 = tr && std::string(tr) == "v1";
         const char* w = std::getenv("DQMC_WRAP");
         c->kron = !(w && std::string(w) == "dense");
+        const char* pf = std::getenv("DQMC_PANEL_FUSE");
+        c->panel_fuse = !(pf && std::string(pf) == "0");
         const char* msp = std::getenv("DQMC_METRO_SPLIT");
         if (msp) {
             const int v = std::atoi(msp);

@@ -175,6 +175,7 @@ struct LookAhead {
     double* V2 = nullptr;
     double* T2 = nullptr;
     int* P2 = nullptr;
+    bool fuse = true;   // panel k + 1 applies update k to its own columns (no separate block-0 launch)
 };
 enum QrPanel : int { QR_PANEL_256 = 0, QR_PANEL_1024 = 1 };
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,

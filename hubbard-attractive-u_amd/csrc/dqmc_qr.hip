@@ -386,10 +386,18 @@ __device__ __forceinline__ void panel_rows_store(double (*tp)[QRB / 2 + 1], cons
 // and read between the two barriers of the column that wrote it, so single buffers suffice.
 // T (H_1..H_nb = I - V T V^T, column-major QRB x QRB) comes from the recursion
 // T(j,i) = -tau_i sum_{k=j}^{i-1} T(j,k) (V^T V)(k,i), thread j forming row j on its own.
+__device__ __forceinline__ void qr_update_tile(double* __restrict__ bufs, long long stride, int N, int W, int b, int cb,
+                                               int p0, int nb, const double* __restrict__ Vs,
+                                               const double* __restrict__ Vrows, long long vr_stride,
+                                               const double* __restrict__ Ts, double (*Wp)[QRB][QRB + 1]);
+// With pp0 >= 0 the workgroup first applies the previous panel's reflectors (V, T of the panel at row
+// pp0, row-major copy pVr) to this panel's columns, as the trailing update's tile 0 would, and then
+// factors them: one launch, and one kernel boundary, fewer per panel.
 __global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs, long long stride, int N, int W,
                                                        int p0, int nb, double* __restrict__ Vs,
                                                        double* __restrict__ Ts, double* __restrict__ Vrows,
-                                                       long long vr_stride)
+                                                       long long vr_stride, int pp0, const double* __restrict__ pV,
+                                                       const double* __restrict__ pT, const double* __restrict__ pVr)
 {
     // latency-bound: issue ahead of the update kernels that share the CU
     __builtin_amdgcn_s_setprio(3);
@@ -406,6 +414,14 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs
     double* A = bufs + (long long)b * stride;
     double* V = Vs + (long long)b * N * QRB;
     double* Tg = Ts + (long long)b * QRB * QRB;
+    static_assert(sizeof(tp) >= 4 * QRB * (QRB + 1) * sizeof(double), "update scratch");
+    if (pp0 >= 0) {
+        // tp is free until the write-back; this workgroup's stores of the tile are visible to it past
+        // the barrier
+        qr_update_tile(bufs, stride, N, W, b, 0, pp0, (N - pp0 < QRB) ? N - pp0 : QRB, pV, pVr, vr_stride, pT,
+                       reinterpret_cast<double(*)[QRB][QRB + 1]>(&tp[0][0]));
+        __syncthreads();
+    }
 
     double x[QRB];
 #pragma unroll
@@ -494,20 +510,17 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs
 //   4. At -= V W2^T on its own tiles (V read column-major: 16 consecutive rows per fragment, the
 //      B-fragments of W2^T from LDS), and stores them.
 // At is read and written exactly once; V and T come from L2.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void qr_update_kernel(double* __restrict__ bufs, long long stride, int N,
-                                                            int W, int batch, int p0, int nb, int cb0, int ncb,
-                                                            const double* __restrict__ Vs,
-                                                            const double* __restrict__ Vrows, long long vr_stride,
-                                                            const double* __restrict__ Ts)
+// The update of column tile cb of chain b by one 256-thread workgroup (all threads call it); Wp is
+// 4 x QRB x (QRB + 1) doubles of LDS.
+__device__ __forceinline__ void qr_update_tile(double* __restrict__ bufs, long long stride, int N, int W, int b, int cb,
+                                               int p0, int nb, const double* __restrict__ Vs,
+                                               const double* __restrict__ Vrows, long long vr_stride,
+                                               const double* __restrict__ Ts, double (*Wp)[QRB][QRB + 1])
 {
-    __shared__ double Wp[4][QRB][QRB + 1];
     typedef Mfma<double>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane >> 4, lc = lane & 15;
     const int R = N - p0, cols = W - (p0 + nb), nrt = (R + 15) >> 4;
-    int b, cb;
-    chain_tile(ncb, batch, b, cb);
-    cb += cb0;
     const int c0 = cb * QRB;
     double* A = bufs + (long long)b * stride;
     const double* V = Vs + (long long)b * N * QRB;
@@ -608,6 +621,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
         }
     }
 }
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void qr_update_kernel(double* __restrict__ bufs, long long stride, int N,
+                                                            int W, int batch, int p0, int nb, int cb0, int ncb,
+                                                            const double* __restrict__ Vs,
+                                                            const double* __restrict__ Vrows, long long vr_stride,
+                                                            const double* __restrict__ Ts)
+{
+    __shared__ double Wp[4][QRB][QRB + 1];
+    int b, cb;
+    chain_tile(ncb, batch, b, cb);
+    qr_update_tile(bufs, stride, N, W, b, cb + cb0, p0, nb, Vs, Vrows, vr_stride, Ts, Wp);
+}
+
 
 // Panel factorization for systems of more than 256 rows (N <= 1024): NW waves per chain, thread t
 // holds the RPT panel rows t, t + 64 NW, .. with all QRB columns in registers.  The column steps,
@@ -742,15 +768,18 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
     // more than 256 rows: the wide panel kernel and the three-GEMM update, in order
     if (N > 256) { update = QR_UPDATE_GEMM; la = nullptr; }
     double* Wt2 = Wt + (long long)batch * wt_stride;
-    auto launch_panel = [&](int p0, int par, hipStream_t st) {
+    // panel at row p0 into the parity-par buffers; with pp0 >= 0 (256-thread panel only) it first applies
+    // the panel at row pp0 (parity ppar) to its own columns
+    auto launch_panel = [&](int p0, int par, hipStream_t st, int pp0 = -1, int ppar = 0) {
         const int nb = (N - p0 < QRB) ? N - p0 : QRB;
         double *V = par ? la->V2 : Vs, *T = par ? la->T2 : Ts, *Vr = par ? Wt2 : Wt;
+        const double *pV = ppar ? la->V2 : Vs, *pT = ppar ? la->T2 : Ts, *pVr = ppar ? Wt2 : Wt;
         if (panel == QR_PANEL_1024)
             hipLaunchKernelGGL(qr_panel4_kernel, dim3(batch), dim3(1024), 0, st, buf, stride, N, W, p0, nb, V, T, Vr,
                                wt_stride);
         else
             hipLaunchKernelGGL(qr_panel_kernel, dim3(batch), dim3(256), 0, st, buf, stride, N, W, p0, nb, V, T, Vr,
-                               wt_stride);
+                               wt_stride, pp0, pV, pT, pVr);
         return hipGetLastError();
     };
     auto launch_update = [&](int p0, int par, int cb0, int ncb, hipStream_t st) {
@@ -761,10 +790,12 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
         return hipGetLastError();
     };
     if (la && la->s2 && update == QR_UPDATE_REG) {
-        // Look-ahead: the update of panel k first covers panel k + 1's columns (block 0); then panel
-        // k + 1 (main stream) runs beside the rest of the update (second stream).  Panel buffers
-        // alternate by parity, so panel k + 1 never overwrites what the rest of update k reads.
+        // Look-ahead: panel k + 1 (main stream) first applies update k to its own columns (block 0) and
+        // then factors them, beside the rest of update k (second stream).  Panel buffers alternate by
+        // parity, so panel k + 1 never overwrites what the rest of update k reads.  The 1024-thread
+        // panel leaves block 0 to its own update launch.
         const int np = (N + QRB - 1) / QRB;
+        const bool fused = panel != QR_PANEL_1024 && la->fuse;
         hipError_t e = launch_panel(0, 0, s);
         if (e != hipSuccess) return e;
         for (int k = 0; k < np; ++k) {
@@ -775,13 +806,13 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
                 if ((e = launch_update(p0, k & 1, 0, ncb, s)) != hipSuccess) return e;
                 continue;
             }
-            if ((e = launch_update(p0, k & 1, 0, 1, s)) != hipSuccess) return e;
+            if (!fused && (e = launch_update(p0, k & 1, 0, 1, s)) != hipSuccess) return e;
             if (ncb > 1) {
                 if ((e = hipEventRecord(la->fork, s)) != hipSuccess) return e;
                 if ((e = hipStreamWaitEvent(la->s2, la->fork, 0)) != hipSuccess) return e;
             }
             // the panel is submitted first so that its workgroups take their slots before the update's
-            if ((e = launch_panel(p0 + QRB, (k + 1) & 1, s)) != hipSuccess) return e;
+            if ((e = launch_panel(p0 + QRB, (k + 1) & 1, s, fused ? p0 : -1, k & 1)) != hipSuccess) return e;
             if (ncb > 1 && (e = launch_update(p0, k & 1, 1, ncb - 1, la->s2)) != hipSuccess) return e;
             if (ncb > 1) {
                 if ((e = hipEventRecord(la->join, la->s2)) != hipSuccess) return e;
@@ -805,7 +836,8 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
                                reg ? Wt : nullptr, wt_stride);
         else
             hipLaunchKernelGGL(qr_panel_kernel, dim3(batch), dim3(256), 0, s, buf, stride, N, W, p0, nb, Vs, Ts,
-                               reg ? Wt : nullptr, wt_stride);
+                               reg ? Wt : nullptr, wt_stride, -1, (const double*)nullptr, (const double*)nullptr,
+                               (const double*)nullptr);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (cols <= 0) continue;
@@ -910,10 +942,18 @@ __device__ __forceinline__ void wave_argmax(double& v, int& i, int lane)
 // below eliminate with the pivot row (read by lane j from LDS, then v_readlane to every lane).
 // The published slots alternate between two buffers by column parity: a slot written for column
 // i + 2 is past the barrier of column i + 1, which every reader of column i reached after its reads.
+__device__ __forceinline__ void lu_update_tile(double* __restrict__ bufs, long long stride, int N, int W, int b, int cb,
+                                               int p0, int nb, const double* __restrict__ Ls,
+                                               const double* __restrict__ Li, const int* __restrict__ Ps,
+                                               double (*S)[QRB][QRB + 1]);
+// With pp0 >= 0 the workgroup first applies the previous panel (L, L11^{-1} and row order of the panel
+// at row pp0) to this panel's columns, as the trailing update's tile 0 would, and then factors them.
 __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs, long long stride, int N, int W,
                                                        int p0, int nb, double* __restrict__ Ls,
                                                        double* __restrict__ Li, int* __restrict__ Ps,
-                                                       uint32_t* __restrict__ health)
+                                                       uint32_t* __restrict__ health, int pp0,
+                                                       const double* __restrict__ pLs, const double* __restrict__ pLi,
+                                                       const int* __restrict__ pPs)
 {
     // latency-bound: issue ahead of the update kernels that share the CU
     __builtin_amdgcn_s_setprio(3);
@@ -930,6 +970,14 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs
     const int R = N - p0;
     const bool live = r < R;
     double* A = bufs + (long long)b * stride;
+    static_assert(sizeof(tp) >= 2 * QRB * (QRB + 1) * sizeof(double), "update scratch");
+    if (pp0 >= 0) {
+        // tp is free until the write-back; this workgroup's stores of the tile are visible to it past
+        // the barrier
+        lu_update_tile(bufs, stride, N, W, b, 0, pp0, (N - pp0 < QRB) ? N - pp0 : QRB, pLs, pLi, pPs,
+                       reinterpret_cast<double(*)[QRB][QRB + 1]>(&tp[0][0]));
+        __syncthreads();
+    }
 
     double x[QRB];
 #pragma unroll
@@ -1033,19 +1081,17 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs
 // reads as B-fragments for A22 -= L21 U12 on its own row tiles.  The tile is read and written once.
 // Four waves per SIMD (at most 128 registers): 145 instead of 164 us for the first panel's update at
 // 256 chains.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void lu_update_kernel(double* __restrict__ bufs, long long stride, int N, int W,
-                                                        int batch, int p0, int nb, int cb0, int ncb,
-                                                        const double* __restrict__ Ls, const double* __restrict__ Li,
-                                                        const int* __restrict__ Ps)
+// The update of column tile cb of chain b by one 256-thread workgroup (all threads call it); S is
+// 2 x QRB x (QRB + 1) doubles of LDS ([0]: A12, [1]: U12).
+__device__ __forceinline__ void lu_update_tile(double* __restrict__ bufs, long long stride, int N, int W, int b, int cb,
+                                               int p0, int nb, const double* __restrict__ Ls,
+                                               const double* __restrict__ Li, const int* __restrict__ Ps,
+                                               double (*S)[QRB][QRB + 1])
 {
-    __shared__ double S[2][QRB][QRB + 1];   // [0]: A12, [1]: U12
     typedef Mfma<double>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int lr = lane >> 4, lc = lane & 15;
     const int R = N - p0, cols = W - (p0 + nb), nrt = (R + 15) >> 4;
-    int b, cb;
-    chain_tile(ncb, batch, b, cb);
-    cb += cb0;
     const int c0 = cb * QRB;
     double* A = bufs + (long long)b * stride;
     const double* L = Ls + (long long)b * N * QRB;
@@ -1115,15 +1161,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     }
 }
 
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void lu_update_kernel(double* __restrict__ bufs, long long stride, int N, int W,
+                                                        int batch, int p0, int nb, int cb0, int ncb,
+                                                        const double* __restrict__ Ls, const double* __restrict__ Li,
+                                                        const int* __restrict__ Ps)
+{
+    __shared__ double S[2][QRB][QRB + 1];
+    int b, cb;
+    chain_tile(ncb, batch, b, cb);
+    lu_update_tile(bufs, stride, N, W, b, cb + cb0, p0, nb, Ls, Li, Ps, S);
+}
+
+
 hipError_t launch_lu_blocked(double* buf, long long stride, int N, int W, int batch, double* Ls, double* Li, int* Ps,
                              uint32_t* health, hipStream_t s, const LookAhead* la)
 {
     if (N > 256 || N < 1 || W <= N) return hipErrorInvalidValue;
     const bool ahead = la && la->s2;
-    auto launch_panel = [&](int p0, int par, hipStream_t st) {
+    // panel at row p0 into the parity-par buffers; with pp0 >= 0 it first applies the panel at row pp0
+    // (parity ppar) to its own columns
+    auto launch_panel = [&](int p0, int par, hipStream_t st, int pp0 = -1, int ppar = 0) {
         const int nb = (N - p0 < QRB) ? N - p0 : QRB;
         hipLaunchKernelGGL(lu_panel_kernel, dim3(batch), dim3(256), 0, st, buf, stride, N, W, p0, nb,
-                           par ? la->V2 : Ls, par ? la->T2 : Li, par ? la->P2 : Ps, health);
+                           par ? la->V2 : Ls, par ? la->T2 : Li, par ? la->P2 : Ps, health, pp0,
+                           (const double*)(ppar ? la->V2 : Ls), (const double*)(ppar ? la->T2 : Li),
+                           (const int*)(ppar ? la->P2 : Ps));
         return hipGetLastError();
     };
     auto launch_update = [&](int p0, int par, int cb0, int ncb, hipStream_t st) {
@@ -1145,14 +1207,14 @@ hipError_t launch_lu_blocked(double* buf, long long stride, int N, int W, int ba
             if (k + 1 < np && (e = launch_panel(p0 + QRB, npar, s)) != hipSuccess) return e;
             continue;
         }
-        // look-ahead, as in launch_qr_blocked: panel k + 1's columns first, then panel k + 1 beside
-        // the rest of the update (panel buffers and row orders alternate by parity)
-        if ((e = launch_update(p0, par, 0, 1, s)) != hipSuccess) return e;
+        // look-ahead, as in launch_qr_blocked: panel k + 1 applies update k to its own columns and
+        // factors them beside the rest of the update (panel buffers and row orders alternate by parity)
+        if (!la->fuse && (e = launch_update(p0, par, 0, 1, s)) != hipSuccess) return e;
         if (ncb > 1) {
             if ((e = hipEventRecord(la->fork, s)) != hipSuccess) return e;
             if ((e = hipStreamWaitEvent(la->s2, la->fork, 0)) != hipSuccess) return e;
         }
-        if ((e = launch_panel(p0 + QRB, npar, s)) != hipSuccess) return e;
+        if ((e = launch_panel(p0 + QRB, npar, s, la->fuse ? p0 : -1, par)) != hipSuccess) return e;
         if (ncb > 1 && (e = launch_update(p0, par, 1, ncb - 1, la->s2)) != hipSuccess) return e;
         if (ncb > 1) {
             if ((e = hipEventRecord(la->join, la->s2)) != hipSuccess) return e;

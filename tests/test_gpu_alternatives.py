@@ -27,6 +27,8 @@ one exercised).
   DQMC_METRO_SPLIT=1|4   every Metropolis launch over all chains, or over four groups of the chains
                          on four streams, instead of two halves on two streams between the points
                          where all chains meet again
+  DQMC_PANEL_FUSE=0      a separate launch for the look-ahead panel's own columns of the previous
+                         trailing update, instead of the panel applying it first
   DQMC_CHAIN=slice       each B-chain factor applied when its slice is done, instead of all factors
                          of a stack block in one pass at its push (also checked at pace 17, where
                          a block takes two passes of at most 16 factors)
@@ -74,7 +76,7 @@ def default_run():
                                      ("DQMC_QR", "v1"), ("DQMC_METRO_THREADS", "256"),
                                      ("DQMC_METRO_THREADS", "512"), ("DQMC_LOOKAHEAD", "0"),
                                      ("DQMC_CHAIN", "slice"), ("DQMC_WRAP_FUSE", "0"), ("DQMC_METRO_SPLIT", "1"),
-                                     ("DQMC_METRO_SPLIT", "4")])
+                                     ("DQMC_METRO_SPLIT", "4"), ("DQMC_PANEL_FUSE", "0")])
 def test_switch_follows_default_chains(default_run, monkeypatch, var, val):
     say("case: %s=%s, 16x16 L=40, 8 chains, 2 sweeps" % (var, val))
     monkeypatch.setenv(var, val)
