@@ -381,7 +381,8 @@ __device__ __forceinline__ void panel_rows_store(double (*tp)[QRB / 2 + 1], cons
 // so N - p0 <= 256) with all QRB columns in registers.  Column i costs two 4-wave barriers: one
 // after the partial norms (alpha = x(i,i) is published with them), one after the partial sums of
 // the 32 products v . x_j (j > i) and v_k . v (k < i, the V^T V entries of T) that one recursive
-// halving reduces per wave.  The column sums reach every lane through v_readlane (uniform values).
+// halving reduces per wave.  The column sums reach every lane through v_readlane (uniform values;
+// an LDS row read at one address per entry measured 66 -> 71 us per panel).
 // Every LDS slot is written between the second barrier of one column and the first of the next,
 // and read between the two barriers of the column that wrote it, so single buffers suffice.
 // T (H_1..H_nb = I - V T V^T, column-major QRB x QRB) comes from the recursion
@@ -939,7 +940,7 @@ __device__ __forceinline__ void wave_argmax(double& v, int& i, int lane)
 // One barrier per column: before it every wave publishes its arg-max (value, row) and that row
 // (values and original index), and row i publishes itself; after it every thread picks the
 // winning wave, row i takes the pivot row, the pivot's old position takes row i, and the rows
-// below eliminate with the pivot row (read by lane j from LDS, then v_readlane to every lane).
+// below eliminate with the pivot row (read from LDS at one address by every lane).
 // The published slots alternate between two buffers by column parity: a slot written for column
 // i + 2 is past the barrier of column i + 1, which every reader of column i reached after its reads.
 __device__ __forceinline__ void lu_update_tile(double* __restrict__ bufs, long long stride, int N, int W, int b, int cb,
@@ -1023,24 +1024,28 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs
             const int ix = redi[par][w];
             if (v > bv || (v == bv && ix < bi)) { bv = v; bi = ix; bw = w; }
         }
-        // pivot row to every lane: lane j reads entry j, v_readlane broadcasts (uniform values)
-        const double pj = (lane < QRB) ? cand[par][bw][lane] : 0.0;
+        // the pivot row, read by every lane at one address (LDS broadcast): 16 loads of two entries
+        // instead of 62 v_readlane of a lane-distributed copy (69.0 -> 66.1 us per panel alone)
+        const double* prow = cand[par][bw];
         if (r == i && bi != i) {
 #pragma unroll
-            for (int m = 0; m < QRB; ++m) x[m] = cand[par][bw][m];
+            for (int m = 0; m < QRB; ++m) x[m] = prow[m];
             org = cando[par][bw];
         } else if (r == bi && bi != i) {
 #pragma unroll
             for (int m = 0; m < QRB; ++m) x[m] = irow[par][m];
             org = irowo[par];
         }
-        const double piv = readlane_f64(pj, i);
+        const double piv = prow[i];
         zero_pivot |= !(piv != 0.0);
         const double l = (live && r > i) ? x[i] / piv : 0.0;
         if (r > i) x[i] = l;
 #pragma unroll
-        for (int m = 0; m < QRB; ++m)
-            if (m > i) x[m] -= l * readlane_f64(pj, m);
+        for (int m = (i + 1) & ~1; m < QRB; m += 2) {
+            const double2 pm = *reinterpret_cast<const double2*>(prow + m);
+            if (m > i) x[m] -= l * pm.x;
+            x[m + 1] -= l * pm.y;
+        }
     }
     if (r == 0 && zero_pivot) atomicOr(health + b, HEALTH_ZERO_PIVOT);
     if (live) {
