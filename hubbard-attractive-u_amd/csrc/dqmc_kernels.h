@@ -187,10 +187,10 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
 // A zero pivot sets HEALTH_ZERO_PIVOT.
 hipError_t launch_lu_blocked(double* buf, long long stride, int N, int W, int batch, double* Ls, double* Li, int* Ps,
                              uint32_t* health, hipStream_t s, const LookAhead* la = nullptr);
-// Blocked in-place back-substitution of the same system: 32-row diagonal solves (thread per
-// right-hand side) and MFMA GEMM updates of the rows above (`tile` = GEMM tile).
+// Blocked in-place back-substitution of the same system: `block`-row diagonal solves (thread per
+// right-hand side; 32, 64 or 128 rows) and MFMA GEMM updates of the rows above (`tile` = GEMM tile).
 hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,
-                               hipStream_t s);
+                               hipStream_t s, int block = 32);
 // In-place X = R^{-1} Z, R the upper triangle of the leading N x N block, Z columns
 // [c0, c0+P) of the same row-major N x W buffer; one-kernel form (DQMC_TRSM=v1).
 hipError_t launch_trsm_upper_v1(double* buf, long long stride, int N, int W, int c0, int P, int batch,

@@ -1231,60 +1231,64 @@ hipError_t launch_lu_blocked(double* buf, long long stride, int N, int W, int ba
 
 // ---------------------------------------------------------------------------------------------
 //  Blocked back-substitution X = R^{-1} Z for the row-major system (R in columns [0, N), Z in
-//  columns [c0, c0+P), solved in place), 32-row blocks, recursively from the bottom half:
+//  columns [c0, c0+P), solved in place), TB-row diagonal blocks, recursively from the bottom half:
 //    trsm_diag_kernel  X_k = R_kk^{-1} Z_k, one thread per right-hand side, R_kk in LDS;
 //    GEMM (MFMA)       Z_lo -= R_{lo, hi} X_hi, as Z^T_{:, lo} -= X_hi^T R_{lo, hi}^T.
+//  The solve is a chain of dependent launches (2 N / TB - 1 of them), so the block size trades the
+//  diagonal kernels' serial work (TB divisions and TB^2 / 2 FMAs per thread) against launches.
 // ---------------------------------------------------------------------------------------------
-#define TRB 32
-
+template <int TB>
 __global__ __launch_bounds__(256) void trsm_diag_kernel(double* __restrict__ bufs, long long stride, int W, int c0,
                                                         int P, int k0, int nbk)
 {
-    __shared__ double Rk[TRB][TRB + 1];
+    // R_kk transposed (Rt[q][m] = R(m, q)), so that the column a step reads is contiguous: two
+    // entries per LDS load, every lane at the same address
+    __shared__ __align__(16) double Rt[TB][TB + 2];
     double* A = bufs + (long long)blockIdx.y * stride;
-    for (int e = threadIdx.x; e < TRB * TRB; e += 256) {
-        const int m = e / TRB, q = e % TRB;
-        Rk[m][q] = (m < nbk && q < nbk) ? A[(long long)(k0 + m) * W + k0 + q] : 0.0;
+    for (int e = threadIdx.x; e < TB * TB; e += 256) {
+        const int m = e / TB, q = e % TB;
+        Rt[q][m] = (m < nbk && q < nbk) ? A[(long long)(k0 + m) * W + k0 + q] : 0.0;
     }
     __syncthreads();
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= P) return;
     double* Z = A + c0 + j;
-    double x[TRB];
+    double x[TB];
 #pragma unroll
-    for (int m = 0; m < TRB; ++m) x[m] = (m < nbk) ? Z[(long long)(k0 + m) * W] : 0.0;
+    for (int m = 0; m < TB; ++m) x[m] = (m < nbk) ? Z[(long long)(k0 + m) * W] : 0.0;
     // Right-looking substitution: once x_m is final, every row above it takes its term at once, so each
-    // step is one division and independent FMAs instead of a dependent chain of up to 31 FMAs.
+    // step is one division and independent FMAs instead of a dependent chain of up to TB - 1 FMAs.
 #pragma unroll
-    for (int m = TRB - 1; m >= 0; --m) {
+    for (int m = TB - 1; m >= 0; --m) {
         if (m < nbk) {
-            const double xm = x[m] / Rk[m][m];
+            const double xm = x[m] / Rt[m][m];
             x[m] = xm;
 #pragma unroll
-            for (int p = 0; p < m; ++p) x[p] -= Rk[p][m] * xm;
+            for (int p = 0; p < m; ++p) x[p] -= Rt[m][p] * xm;
         }
     }
 #pragma unroll
-    for (int m = 0; m < TRB; ++m)
+    for (int m = 0; m < TB; ++m)
         if (m < nbk) Z[(long long)(k0 + m) * W] = x[m];
 }
 
-// Blocks [lo, hi) of 32 rows: the lower half of the range is updated with the solved upper half in
+// Blocks [lo, hi) of TB rows: the lower half of the range is updated with the solved upper half in
 // one GEMM, Z_{lo:mid} -= R_{lo:mid, mid:hi} X_{mid:hi}, so each level of the recursion passes over half of
-// Z once (three passes at N = 256, against seven rank-32 updates of growing height bottom-up).
+// Z once (log2(N / TB) passes, against N / TB - 1 rank-TB updates of growing height bottom-up).
+template <int TB>
 static hipError_t trsm_range(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile, int lo,
                              int hi, hipStream_t s)
 {
     if (hi - lo == 1) {
-        const int k0 = lo * TRB, nbk = (N - k0 < TRB) ? N - k0 : TRB;
-        hipLaunchKernelGGL(trsm_diag_kernel, dim3((P + 255) / 256, batch), dim3(256), 0, s, buf, stride, W, c0, P, k0,
-                           nbk);
+        const int k0 = lo * TB, nbk = (N - k0 < TB) ? N - k0 : TB;
+        hipLaunchKernelGGL(trsm_diag_kernel<TB>, dim3((P + 255) / 256, batch), dim3(256), 0, s, buf, stride, W, c0, P,
+                           k0, nbk);
         return hipGetLastError();
     }
     const int mid = (lo + hi) / 2;
-    hipError_t e = trsm_range(buf, stride, N, W, c0, P, batch, tile, mid, hi, s);
+    hipError_t e = trsm_range<TB>(buf, stride, N, W, c0, P, batch, tile, mid, hi, s);
     if (e != hipSuccess) return e;
-    const int r0 = lo * TRB, s0 = mid * TRB, s1 = (hi * TRB < N) ? hi * TRB : N;
+    const int r0 = lo * TB, s0 = mid * TB, s1 = (hi * TB < N) ? hi * TB : N;
     GemmArgs<double> g;
     g.M = P; g.Nc = s0 - r0; g.K = s1 - s0;
     g.A = buf + c0 + (long long)s0 * W; g.sA = stride; g.lda = W; g.tA = 0;   // X_{mid:hi}^T
@@ -1292,14 +1296,19 @@ static hipError_t trsm_range(double* buf, long long stride, int N, int W, int c0
     g.C = buf + c0 + (long long)r0 * W; g.sC = stride; g.ldc = W; g.tC = 0;   // Z_{lo:mid}^T
     g.alpha = -1.0; g.beta = 1.0; g.batch = batch;
     if ((e = launch_gemm<double>(g, tile, s)) != hipSuccess) return e;
-    return trsm_range(buf, stride, N, W, c0, P, batch, tile, lo, mid, s);
+    return trsm_range<TB>(buf, stride, N, W, c0, P, batch, tile, lo, mid, s);
 }
 
 hipError_t launch_trsm_blocked(double* buf, long long stride, int N, int W, int c0, int P, int batch, int tile,
-                               hipStream_t s)
+                               hipStream_t s, int block)
 {
     if (N < 1 || P < 1 || c0 < N || c0 + P > W) return hipErrorInvalidValue;
-    return trsm_range(buf, stride, N, W, c0, P, batch, tile, 0, (N + TRB - 1) / TRB, s);
+    switch (block) {
+    case 32: return trsm_range<32>(buf, stride, N, W, c0, P, batch, tile, 0, (N + 31) / 32, s);
+    case 64: return trsm_range<64>(buf, stride, N, W, c0, P, batch, tile, 0, (N + 63) / 64, s);
+    case 128: return trsm_range<128>(buf, stride, N, W, c0, P, batch, tile, 0, (N + 127) / 128, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
