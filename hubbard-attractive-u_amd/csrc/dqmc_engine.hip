@@ -72,6 +72,9 @@ struct dqmc_ctx {
     // diagonal block of the blocked back-substitution (DQMC_TRSM_BLOCK; 0: 64 rows from N = 64 up, else
     // 32; at N = 256, 256 chains: 348 us with 64, 402 with 32, 468 with 128 for 256 right-hand sides)
     int trsm_block = 0;
+    // the recompose system's two products and right-hand side in one launch (DQMC_RECOMPOSE_FUSE=0: two
+    // GEMMs and a copy)
+    bool recompose_fuse = true;
     int qr_update = QR_UPDATE_REG;   // QR trailing update (DQMC_QR_UPDATE=gemm: three GEMMs)
     int qr_panel = QR_PANEL_256;     // QR panel: 4 waves per chain (DQMC_QR_PANEL=1024: 16 waves)
     bool solve_lu = true;            // recompose solve by partial-pivot LU (DQMC_SOLVE=qr: Householder QR)
@@ -624,6 +627,41 @@ int solve_00bb(dqmc_ctx* c, const double* u, const double* v, const double* d, d
     return 0;
 }
 
+// The recompose system in qr (row-major N x W): columns [0, N) hold
+//   temp = Dlmax^{-1} (Ul^T Ur) Drmax^{-1} + Dlmin (Vl^T Vr) Drmin   (stable_numerics.hpp:220-233)
+// and columns [N, 2N) the right-hand side Dlmax^{-1} Ul^T.  With the 128 x 128 FP64 tile both products
+// are one launch over K = 2N that also stores the right-hand side from its staged operand; otherwise
+// two GEMMs and a copy.
+int recompose_system(dqmc_ctx* c, const double* Ul, const double* Vl, const double* Dl, const double* Ur,
+                     const double* Vr, const double* Dr, int W)
+{
+    const long long st = (long long)c->S * c->NN(), sd = (long long)c->S * c->N;
+    const int N = c->N;
+    if (c->recompose_fuse && c->tile64 == TILE_128x128 && N >= 128) {
+        GemmArgs<double> g;
+        g.M = N; g.Nc = N; g.K = N;
+        g.A = Ul; g.sA = st; g.lda = N; g.tA = 1;
+        g.B = Ur; g.sB = st; g.ldb = N; g.tB = 0;
+        g.C = c->qr; g.sC = c->sqr(); g.ldc = W; g.tC = 1;
+        g.alpha = 1.0; g.beta = 0.0;
+        g.rowd = dvec(DIAG_INVMAX1, Dl, sd); g.cold = dvec(DIAG_INVMAX1, Dr, sd);
+        g.batch = c->B;
+        GemmPair p;
+        p.A2 = Vl; p.sA2 = st; p.lda2 = N;
+        p.B2 = Vr; p.sB2 = st; p.ldb2 = N;
+        p.rowd2 = dvec(DIAG_MIN1, Dl, sd); p.cold2 = dvec(DIAG_MIN1, Dr, sd);
+        p.R = c->qr + N; p.sR = c->sqr(); p.ldr = W;
+        TIMED(c, KT_GEMM, launch_gemm_pair(g, p, c->stream));
+        return 0;
+    }
+    RCHECK(gemm<double>(c, Ul, st, N, 1, Ur, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_INVMAX1, Dl, sd), dnone(),
+                        dvec(DIAG_INVMAX1, Dr, sd)));
+    RCHECK(gemm<double>(c, Vl, st, N, 1, Vr, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_MIN1, Dl, sd), dnone(),
+                        dvec(DIAG_MIN1, Dr, sd), 1.0, 1.0));
+    HCHECK(c, launch_build_rhs(Ul, st, dvec(DIAG_INVMAX1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, N, c->B, c->stream));
+    return 0;
+}
+
 // Equal-time G from the stack tops into `out` (compute_equaltime_gf, stable_numerics.hpp:186-239).
 int recompute_eq(dqmc_ctx* c, double* out)
 {
@@ -637,12 +675,9 @@ int recompute_eq(dqmc_ctx* c, double* out)
     if (Rs.len == 0) return solve_00bb(c, sU(c, Ls, Ls.len - 1), sV(c, Ls, Ls.len - 1), sD(c, Ls, Ls.len - 1), out, false);
     const double *Ul = sU(c, Ls, Ls.len - 1), *Vl = sV(c, Ls, Ls.len - 1), *Dl = sD(c, Ls, Ls.len - 1);
     const double *Ur = sU(c, Rs, Rs.len - 1), *Vr = sV(c, Rs, Rs.len - 1), *Dr = sD(c, Rs, Rs.len - 1);
-    // temp = Dlmax^{-1} (Ul^T Ur) Drmax^{-1} + Dlmin (Vl^T Vr) Drmin, written row-major into qr
-    RCHECK(gemm<double>(c, Ul, st, N, 1, Ur, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_INVMAX1, Dl, sd), dnone(),
-                        dvec(DIAG_INVMAX1, Dr, sd)));
-    RCHECK(gemm<double>(c, Vl, st, N, 1, Vr, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_MIN1, Dl, sd), dnone(),
-                        dvec(DIAG_MIN1, Dr, sd), 1.0, 1.0));
-    HCHECK(c, launch_build_rhs(Ul, st, dvec(DIAG_INVMAX1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, N, c->B, c->stream));
+    // temp = Dlmax^{-1} (Ul^T Ur) Drmax^{-1} + Dlmin (Vl^T Vr) Drmin, written row-major into qr, and the
+    // right-hand side Dlmax^{-1} Ul^T beside it
+    RCHECK(recompose_system(c, Ul, Vl, Dl, Ur, Vr, Dr, W));
     TIMED(c, KT_QR, factor_any(c, W));
     TIMED(c, KT_TRSM, trsm_any(c, c->qr, c->sqr(), N, W, N, N, c->B, c->stream));
     // G = Ur Drmax^{-1} X
@@ -671,11 +706,7 @@ int recompute_dyn(dqmc_ctx* c, double* gtt, double* gt0n, double* g0tn)
     const double *Ul = sU(c, Ls, Ls.len - 1), *Vl = sV(c, Ls, Ls.len - 1), *Dl = sD(c, Ls, Ls.len - 1);
     const double *Ur = sU(c, Rs, Rs.len - 1), *Vr = sV(c, Rs, Rs.len - 1), *Dr = sD(c, Rs, Rs.len - 1);
     const int W = c->wdyn();
-    RCHECK(gemm<double>(c, Ul, st, N, 1, Ur, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_INVMAX1, Dl, sd), dnone(),
-                        dvec(DIAG_INVMAX1, Dr, sd)));
-    RCHECK(gemm<double>(c, Vl, st, N, 1, Vr, st, N, 0, c->qr, c->sqr(), W, 1, dvec(DIAG_MIN1, Dl, sd), dnone(),
-                        dvec(DIAG_MIN1, Dr, sd), 1.0, 1.0));
-    HCHECK(c, launch_build_rhs(Ul, st, dvec(DIAG_INVMAX1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, N, c->B, c->stream));
+    RCHECK(recompose_system(c, Ul, Vl, Dl, Ur, Vr, Dr, W));
     HCHECK(c, launch_build_rhs(Vl, st, dvec(DIAG_MIN1, Dl, sd), 1.0, c->qr, c->sqr(), N, W, 2 * N, c->B, c->stream));
     TIMED(c, KT_QR, factor_any(c, W));
     TIMED(c, KT_TRSM, trsm_any(c, c->qr, c->sqr(), N, W, N, 2 * N, c->B, c->stream));
@@ -1053,6 +1084,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         }
         const char* w = std::getenv("DQMC_WRAP");
         c->kron = !(w && std::string(w) == "dense");
+        const char* rf = std::getenv("DQMC_RECOMPOSE_FUSE");
+        c->recompose_fuse = !(rf && std::string(rf) == "0");
         const char* pf = std::getenv("DQMC_PANEL_FUSE");
         c->panel_fuse = !(pf && std::string(pf) == "0");
         const char* msp = std::getenv("DQMC_METRO_SPLIT");

@@ -245,13 +245,21 @@ __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs<T> g)
 // so every LDS fragment feeds four MFMAs and every global byte eight times the flops of the 64 x 64
 // tile.  K advances by 8 through double-buffered LDS, with the global loads of the next step in one
 // register stage, so that two workgroups fit a CU.
-template <bool TA, bool TB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm128_kernel(GemmArgs<double> g)
+// With PAIR the tile is the sum of two products that share the shape and transposes,
+//   C = diag(r1) op(A) op(B) diag(c1) + diag(r2) op(A2) op(B2) diag(c2)
+// (g.rowd, g.cold and the pair's rowd2, cold2), as one K loop over both: the row scalings are applied to
+// the A operands and the column scalings to the B operands as they are stored in LDS.  When p.R is
+// set, the tiles of column block 0 also store their scaled first A operand, diag(r1) op(A), row-major
+// at p.R (row i at p.R + i * p.ldr).
+template <bool TA, bool TB, bool PAIR>
+__device__ __forceinline__ void gemm128_body(const GemmArgs<double>& g, const GemmPair& p)
 {
     constexpr int BM = 128, BN = 128, BK = 8, WM = 64, WN = 64, NA = BM * BK / 256, NB = BN * BK / 256;
+    constexpr int NP = PAIR ? 2 : 1;
     __shared__ double As[2][BK][BM + 1];
     __shared__ double Bs[2][BK][BN + 1];
-    __shared__ double sk[256], srow[BM], scol[BN];   // diagonal scalings, evaluated once per tile
+    // diagonal scalings, evaluated once per tile (PAIR: per product, applied to the operands)
+    __shared__ double sk[256], srow[NP][BM], scol[NP][BN];
     typedef Mfma<double>::acc_t acc_t;
 
     const int gm = (g.M + BM - 1) / BM, gn = (g.Nc + BN - 1) / BN;
@@ -262,13 +270,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     const int wm = wave >> 1, wn = wave & 1;
     const double* __restrict__ A = g.A + (long long)b * g.sA;
     const double* __restrict__ B = g.B + (long long)b * g.sB;
+    const double* __restrict__ A2 = PAIR ? p.A2 + (long long)b * p.sA2 : nullptr;
+    const double* __restrict__ B2 = PAIR ? p.B2 + (long long)b * p.sB2 : nullptr;
+    double* __restrict__ R = (PAIR && p.R && n0 == 0) ? p.R + (long long)b * p.sR : nullptr;
     double* __restrict__ C = g.C + (long long)b * g.sC;
-    const bool innd = g.innd.mode != DIAG_NONE;
+    const bool innd = !PAIR && g.innd.mode != DIAG_NONE;
     if (innd)
         for (int kk = tid; kk < g.K && kk < 256; kk += 256) sk[kk] = diag_value(g.innd, b, kk);
-    if (tid < BM) srow[tid] = (g.rowd.mode != DIAG_NONE && m0 + tid < g.M) ? diag_value(g.rowd, b, m0 + tid) : 1.0;
-    else
-        scol[tid - BM] = (g.cold.mode != DIAG_NONE && n0 + tid - BM < g.Nc) ? diag_value(g.cold, b, n0 + tid - BM) : 1.0;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+        const DiagArg& rd = q ? p.rowd2 : g.rowd;
+        const DiagArg& cd = q ? p.cold2 : g.cold;
+        if (tid < BM) srow[q][tid] = (rd.mode != DIAG_NONE && m0 + tid < g.M) ? diag_value(rd, b, m0 + tid) : 1.0;
+        else scol[q][tid - BM] = (cd.mode != DIAG_NONE && n0 + tid - BM < g.Nc) ? diag_value(cd, b, n0 + tid - BM) : 1.0;
+    }
     __syncthreads();
 
     acc_t acc[4][4];
@@ -277,35 +292,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    auto fetch = [&](int k0, double (&ra)[NA], double (&rb)[NB]) {
+    // step ks covers k0 = (ks mod nk1) * BK of product ks / nk1
+    const int nk1 = (g.K + BK - 1) / BK;
+    auto fetch = [&](int ks, double (&ra)[NA], double (&rb)[NB]) {
+        const bool second = PAIR && ks >= nk1;
+        const double* __restrict__ Aq = second ? A2 : A;
+        const double* __restrict__ Bq = second ? B2 : B;
+        const int lda = second ? p.lda2 : g.lda, ldb = second ? p.ldb2 : g.ldb;
+        const int k0 = (second ? ks - nk1 : ks) * BK;
 #pragma unroll
         for (int u = 0; u < NA; ++u) {
             const int e = u * 256 + tid;
             const int i = TA ? e / BK : e % BM, kk = TA ? e % BK : e / BM;
             const int gi = m0 + i, gk = k0 + kk;
-            ra[u] = (gi < g.M && gk < g.K) ? (TA ? A[gk + (long long)gi * g.lda] : A[gi + (long long)gk * g.lda]) : 0.0;
+            ra[u] = (gi < g.M && gk < g.K) ? (TA ? Aq[gk + (long long)gi * lda] : Aq[gi + (long long)gk * lda]) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             const int e = u * 256 + tid;
             const int j = TB ? e % BN : e / BK, kk = TB ? e / BN : e % BK;
             const int gj = n0 + j, gk = k0 + kk;
-            rb[u] = (gj < g.Nc && gk < g.K) ? (TB ? B[gj + (long long)gk * g.ldb] : B[gk + (long long)gj * g.ldb]) : 0.0;
+            rb[u] = (gj < g.Nc && gk < g.K) ? (TB ? Bq[gj + (long long)gk * ldb] : Bq[gk + (long long)gj * ldb]) : 0.0;
         }
     };
-    auto stash = [&](int buf, int k0, const double (&ra)[NA], const double (&rb)[NB]) {
+    auto stash = [&](int buf, int ks, const double (&ra)[NA], const double (&rb)[NB]) {
+        const int q = (PAIR && ks >= nk1) ? 1 : 0;
+        const int k0 = (q ? ks - nk1 : ks) * BK;
 #pragma unroll
         for (int u = 0; u < NA; ++u) {
             const int e = u * 256 + tid;
-            const int kk = TA ? e % BK : e / BM;
+            const int kk = TA ? e % BK : e / BM, i = TA ? e / BK : e % BM;
             double v = ra[u];
             if (innd && k0 + kk < g.K) v *= sk[k0 + kk];
-            As[buf][kk][TA ? e / BK : e % BM] = v;
+            if (PAIR) {
+                v *= srow[q][i];
+                if (R && q == 0 && m0 + i < g.M && k0 + kk < g.K) R[(long long)(m0 + i) * p.ldr + k0 + kk] = v;
+            }
+            As[buf][kk][i] = v;
         }
 #pragma unroll
         for (int u = 0; u < NB; ++u) {
             const int e = u * 256 + tid;
-            Bs[buf][TB ? e / BN : e % BK][TB ? e % BN : e / BK] = rb[u];
+            const int j = TB ? e % BN : e / BK;
+            double v = rb[u];
+            if (PAIR) v *= scol[q][j];
+            Bs[buf][TB ? e / BN : e % BK][j] = v;
         }
     };
     auto compute = [&](int buf) {
@@ -323,16 +354,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
         }
     };
 
-    const int nk = (g.K + BK - 1) / BK;
+    const int nk = NP * nk1;
     double ra[NA], rb[NB];
     fetch(0, ra, rb);
     stash(0, 0, ra, rb);
     __syncthreads();
     // one register stage: step ks + 1 is loaded while step ks is multiplied from LDS buffer ks & 1
     for (int ks = 0; ks < nk; ++ks) {
-        if (ks + 1 < nk) fetch((ks + 1) * BK, ra, rb);
+        if (ks + 1 < nk) fetch(ks + 1, ra, rb);
         compute(ks & 1);
-        if (ks + 1 < nk) stash((ks + 1) & 1, (ks + 1) * BK, ra, rb);
+        if (ks + 1 < nk) stash((ks + 1) & 1, ks + 1, ra, rb);
         __syncthreads();
     }
 
@@ -346,8 +377,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const int gj = n0 + wn * WN + j * 16 + (lane & 15);
                 if (gi < g.M && gj < g.Nc) {
                     double v = acc[i][j][r] * g.alpha;
-                    if (g.rowd.mode != DIAG_NONE) v *= srow[gi - m0];
-                    if (g.cold.mode != DIAG_NONE) v *= scol[gj - n0];
+                    if (!PAIR && g.rowd.mode != DIAG_NONE) v *= srow[0][gi - m0];
+                    if (!PAIR && g.cold.mode != DIAG_NONE) v *= scol[0][gj - n0];
                     const long long off = g.tC ? (gj + (long long)gi * g.ldc) : (gi + (long long)gj * g.ldc);
                     if (g.beta != 0.0) v += g.beta * C[off];
                     C[off] = v;
@@ -356,11 +387,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 }
 
 template <bool TA, bool TB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm128_kernel(GemmArgs<double> g)
+{
+    gemm128_body<TA, TB, false>(g, GemmPair());
+}
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm128_pair_kernel(GemmArgs<double> g,
+                                                                                                  GemmPair p)
+{
+    gemm128_body<TA, TB, true>(g, p);
+}
+
+template <bool TA, bool TB>
 static hipError_t launch_gemm128(const GemmArgs<double>& a, hipStream_t s)
 {
     const long long blocks = (long long)((a.M + 127) / 128) * ((a.Nc + 127) / 128) * a.batch;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL((gemm128_kernel<TA, TB>), dim3((unsigned)blocks), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_pair(const GemmArgs<double>& a, const GemmPair& p, hipStream_t s)
+{
+    if (a.batch <= 0 || a.M <= 0 || a.Nc <= 0) return hipSuccess;
+    if (a.M < 128 || a.Nc < 128 || a.K < 64 || a.innd.mode != DIAG_NONE || a.beta != 0.0 || a.tA != 1 || a.tB != 0)
+        return hipErrorInvalidValue;
+    const long long blocks = (long long)((a.M + 127) / 128) * ((a.Nc + 127) / 128) * a.batch;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm128_pair_kernel<true, false>), dim3((unsigned)blocks), dim3(256), 0, s, a, p);
     return hipGetLastError();
 }
 

@@ -112,6 +112,18 @@ enum GemmTile : int { TILE_64x64 = 0, TILE_32x32 = 1, TILE_64x32 = 2, TILE_64x64
 
 template <typename T>
 hipError_t launch_gemm(const GemmArgs<T>& a, int tile, hipStream_t s);
+// The second product of a two-product FP64 GEMM (same shape and transposes as the first, K each),
+//   C = diag(a.rowd) op(A) op(B) diag(a.cold) + diag(rowd2) op(A2) op(B2) diag(cold2),
+// and the optional row-major copy R (ld ldr) of the first scaled operand diag(a.rowd) op(A).
+struct GemmPair {
+    const double* A2 = nullptr; long long sA2 = 0; int lda2 = 0;
+    const double* B2 = nullptr; long long sB2 = 0; int ldb2 = 0;
+    DiagArg rowd2, cold2;
+    double* R = nullptr; long long sR = 0; int ldr = 0;
+};
+// One 128 x 128-tile launch of the two products (op(A) = A^T, op(B) = B, beta = 0, no inner scaling;
+// M, Nc >= 128 and K >= 64), the scalings applied to the operands in LDS.
+hipError_t launch_gemm_pair(const GemmArgs<double>& a, const GemmPair& p, hipStream_t s);
 
 // Metropolis sweep of one slice with delayed rank-k updates.
 template <typename T>
