@@ -72,8 +72,9 @@ struct dqmc_ctx {
     // diagonal block of the blocked back-substitution (DQMC_TRSM_BLOCK; 0: 64 rows from N = 64 up, else
     // 32; at N = 256, 256 chains: 348 us with 64, 402 with 32, 468 with 128 for 256 right-hand sides)
     int trsm_block = 0;
-    // the recompose system's two products and right-hand side in one launch (DQMC_RECOMPOSE_FUSE=0: two
-    // GEMMs and a copy)
+    // the recompose system's two products and right-hand side in one launch, and the stabilization's
+    // compare-and-commit in the epilogue of the last product (DQMC_RECOMPOSE_FUSE=0: two GEMMs and a
+    // copy, and the compare-and-commit kernel)
     bool recompose_fuse = true;
     int qr_update = QR_UPDATE_REG;   // QR trailing update (DQMC_QR_UPDATE=gemm: three GEMMs)
     int qr_panel = QR_PANEL_256;     // QR panel: 4 waves per chain (DQMC_QR_PANEL=1024: 16 waves)
@@ -166,7 +167,7 @@ struct dqmc_ctx {
     long long eq_count = 0, dyn_count = 0;
     double *meas_cache = nullptr, *meas_data = nullptr, *meas_part = nullptr, *costab = nullptr;
     double* wd_bound = nullptr;      // watchdog bound on max|G_recomputed - G_working| (0 = off)
-    double* cmp_part = nullptr;      // [B][16][3] partial reductions of compare_commit
+    double* cmp_part = nullptr;      // [B][CMP_PART_SLOTS][3] partial reductions of compare_commit
     DStack left, right;
     // parameter adjoint at fixed fields (allocated on first use)
     int* nb = nullptr;
@@ -663,7 +664,10 @@ int recompose_system(dqmc_ctx* c, const double* Ul, const double* Vl, const doub
 }
 
 // Equal-time G from the stack tops into `out` (compute_equaltime_gf, stable_numerics.hpp:186-239).
-int recompute_eq(dqmc_ctx* c, double* out)
+// With `committed` set, the general case may instead compare the new G with the working G64 and commit it
+// in the epilogue of its last product (the wrap error and health folded as launch_compare_commit does);
+// *committed then says so, and `out` is not written.
+int recompute_eq(dqmc_ctx* c, double* out, bool* committed = nullptr)
 {
     const long long nn = c->NN(), st = (long long)c->S * nn, sd = (long long)c->S * c->N;
     const int N = c->N, W = c->weq();
@@ -681,6 +685,23 @@ int recompute_eq(dqmc_ctx* c, double* out)
     TIMED(c, KT_QR, factor_any(c, W));
     TIMED(c, KT_TRSM, trsm_any(c, c->qr, c->sqr(), N, W, N, N, c->B, c->stream));
     // G = Ur Drmax^{-1} X
+    if (committed && c->recompose_fuse && c->prec != DQMC_MIXED && c->tile64 == TILE_128x128 && N >= 128 &&
+        N <= 256) {
+        GemmArgs<double> g;
+        g.M = N; g.Nc = N; g.K = N;
+        g.A = Ur; g.sA = st; g.lda = N; g.tA = 0;
+        g.B = c->qr + N; g.sB = c->sqr(); g.ldb = W; g.tB = 1;
+        g.C = c->G64; g.sC = nn; g.ldc = N; g.tC = 0;
+        g.alpha = 1.0; g.beta = 0.0;
+        g.innd = dvec(DIAG_INVMAX1, Dr, sd);
+        g.batch = c->B;
+        TIMED(c, KT_GEMM, launch_gemm_commit(g, c->cmp_part, CMP_PART_SLOTS, c->stream));
+        const int tiles = ((N + 127) / 128) * ((N + 127) / 128);
+        HCHECK(c, launch_compare_fold(c->cmp_part, tiles, c->wrap_eq, c->maxdG, c->wd_bound, c->health, c->B,
+                                      c->stream));
+        *committed = true;
+        return 0;
+    }
     RCHECK(gemm<double>(c, Ur, st, N, 0, c->qr + N, c->sqr(), W, 1, out, nn, N, 0, dnone(), dvec(DIAG_INVMAX1, Dr, sd)));
     return 0;
 }
@@ -736,9 +757,11 @@ int store_current(dqmc_ctx* c, double* vec, int t)
 int stabilize_eq(dqmc_ctx* c)
 {
     RCHECK(metro_join(c));
-    RCHECK(recompute_eq(c, c->W3));
-    HCHECK(c, launch_compare_commit(c->W3, c->G64, c->prec == DQMC_MIXED ? c->G32 : nullptr, c->NN(), c->N, c->wrap_eq,
-                                    c->maxdG, c->wd_bound, c->health, c->cmp_part, c->B, c->stream));
+    bool committed = false;
+    RCHECK(recompute_eq(c, c->W3, &committed));
+    if (!committed)
+        HCHECK(c, launch_compare_commit(c->W3, c->G64, c->prec == DQMC_MIXED ? c->G32 : nullptr, c->NN(), c->N,
+                                        c->wrap_eq, c->maxdG, c->wd_bound, c->health, c->cmp_part, c->B, c->stream));
     return 0;
 }
 
@@ -1173,7 +1196,7 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     RCHECK(c->alloc(&c->wrap_eq, (size_t)B * 8)); RCHECK(c->alloc(&c->wrap_dyn, (size_t)B * 8));
     RCHECK(c->alloc(&c->maxdG, (size_t)B * 8));
     RCHECK(c->alloc(&c->health, (size_t)B * 4)); RCHECK(c->alloc(&c->wd_bound, 8));
-    RCHECK(c->alloc(&c->cmp_part, (size_t)B * 16 * 3 * 8));
+    RCHECK(c->alloc(&c->cmp_part, (size_t)B * CMP_PART_SLOTS * 3 * 8));
 
     // constant tables
     std::vector<double> ek, iek;

@@ -251,10 +251,15 @@ __global__ __launch_bounds__(256) void gemm64_kernel(GemmArgs<T> g)
 // the A operands and the column scalings to the B operands as they are stored in LDS.  When p.R is
 // set, the tiles of column block 0 also store their scaled first A operand, diag(r1) op(A), row-major
 // at p.R (row i at p.R + i * p.ldr).
-template <bool TA, bool TB, bool PAIR>
-__device__ __forceinline__ void gemm128_body(const GemmArgs<double>& g, const GemmPair& p)
+// With COMMIT the tile is stored over the previous contents of C after comparing with them: its signed
+// maximum and maximum absolute difference (new - old) and a non-finite flag go to
+// part[(chain * tiles + tile) * 3 + {0, 1, 2}], to be folded per chain by launch_compare_fold.
+enum { G128_PLAIN = 0, G128_PAIR = 1, G128_COMMIT = 2 };
+template <bool TA, bool TB, int MODE>
+__device__ __forceinline__ void gemm128_body(const GemmArgs<double>& g, const GemmPair& p, double* __restrict__ part)
 {
     constexpr int BM = 128, BN = 128, BK = 8, WM = 64, WN = 64, NA = BM * BK / 256, NB = BN * BK / 256;
+    constexpr bool PAIR = MODE == G128_PAIR, COMMIT = MODE == G128_COMMIT;
     constexpr int NP = PAIR ? 2 : 1;
     __shared__ double As[2][BK][BM + 1];
     __shared__ double Bs[2][BK][BN + 1];
@@ -367,6 +372,8 @@ __device__ __forceinline__ void gemm128_body(const GemmArgs<double>& g, const Ge
         __syncthreads();
     }
 
+    double es = -DBL_MAX, ea = 0.0;
+    bool bad = false;
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -380,23 +387,54 @@ __device__ __forceinline__ void gemm128_body(const GemmArgs<double>& g, const Ge
                     if (!PAIR && g.rowd.mode != DIAG_NONE) v *= srow[0][gi - m0];
                     if (!PAIR && g.cold.mode != DIAG_NONE) v *= scol[0][gj - n0];
                     const long long off = g.tC ? (gj + (long long)gi * g.ldc) : (gi + (long long)gj * g.ldc);
-                    if (g.beta != 0.0) v += g.beta * C[off];
+                    if (COMMIT) {
+                        const double old = C[off], d = v - old;
+                        es = fmax(es, d);
+                        ea = fmax(ea, fabs(d));
+                        bad |= !isfinite(v) || !isfinite(old);
+                    } else if (g.beta != 0.0) {
+                        v += g.beta * C[off];
+                    }
                     C[off] = v;
                 }
             }
+    if constexpr (COMMIT) {
+        // the tile's maxima over its four waves (max is exact: the order of the pairings is free)
+        __shared__ double red[3][4];
+        double fl = bad ? 1.0 : 0.0;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            es = fmax(es, __shfl_xor(es, o, 64));
+            ea = fmax(ea, __shfl_xor(ea, o, 64));
+            fl = fmax(fl, __shfl_xor(fl, o, 64));
+        }
+        if (lane == 0) { red[0][wave] = es; red[1][wave] = ea; red[2][wave] = fl; }
+        __syncthreads();
+        if (tid < 3) {
+            const double m = fmax(fmax(red[tid][0], red[tid][1]), fmax(red[tid][2], red[tid][3]));
+            part[((long long)b * gm * gn + t) * 3 + tid] = m;
+        }
+    }
 }
 
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm128_kernel(GemmArgs<double> g)
 {
-    gemm128_body<TA, TB, false>(g, GemmPair());
+    gemm128_body<TA, TB, G128_PLAIN>(g, GemmPair(), nullptr);
 }
 
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm128_pair_kernel(GemmArgs<double> g,
                                                                                                   GemmPair p)
 {
-    gemm128_body<TA, TB, true>(g, p);
+    gemm128_body<TA, TB, G128_PAIR>(g, p, nullptr);
+}
+
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void gemm128_commit_kernel(GemmArgs<double> g,
+                                                                                                    double* part)
+{
+    gemm128_body<TA, TB, G128_COMMIT>(g, GemmPair(), part);
 }
 
 template <bool TA, bool TB>
@@ -416,6 +454,18 @@ hipError_t launch_gemm_pair(const GemmArgs<double>& a, const GemmPair& p, hipStr
     const long long blocks = (long long)((a.M + 127) / 128) * ((a.Nc + 127) / 128) * a.batch;
     if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL((gemm128_pair_kernel<true, false>), dim3((unsigned)blocks), dim3(256), 0, s, a, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_commit(const GemmArgs<double>& a, double* part, int max_tiles, hipStream_t s)
+{
+    if (a.batch <= 0 || a.M <= 0 || a.Nc <= 0) return hipSuccess;
+    if (a.M < 128 || a.Nc < 128 || a.K < 64 || (a.innd.mode != DIAG_NONE && a.K > 256) || a.beta != 0.0 ||
+        a.tA != 0 || a.tB != 1)
+        return hipErrorInvalidValue;
+    const long long tiles = (long long)((a.M + 127) / 128) * ((a.Nc + 127) / 128);
+    if (tiles > max_tiles || tiles * a.batch > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm128_commit_kernel<false, true>), dim3((unsigned)(tiles * a.batch)), dim3(256), 0, s, a, part);
     return hipGetLastError();
 }
 

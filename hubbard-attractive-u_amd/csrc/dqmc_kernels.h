@@ -124,6 +124,11 @@ struct GemmPair {
 // One 128 x 128-tile launch of the two products (op(A) = A^T, op(B) = B, beta = 0, no inner scaling;
 // M, Nc >= 128 and K >= 64), the scalings applied to the operands in LDS.
 hipError_t launch_gemm_pair(const GemmArgs<double>& a, const GemmPair& p, hipStream_t s);
+// C = op(A) diag(innd) op(B) (op(A) = A, op(B) = B^T, 128 x 128 FP64 tile, at most max_tiles tiles per chain)
+// stored over the previous C after comparing with it: per tile the signed maximum and maximum absolute
+// difference (new - old) and a non-finite flag, into part (chain-major, 3 doubles per tile), which
+// launch_compare_fold reduces with np = tiles per chain.
+hipError_t launch_gemm_commit(const GemmArgs<double>& a, double* part, int max_tiles, hipStream_t s);
 
 // Metropolis sweep of one slice with delayed rank-k updates.
 template <typename T>
@@ -238,10 +243,15 @@ hipError_t launch_extract(const double* buf, long long sbuf, int W, int c0, doub
 // non-null).  err_signed / err_abs are per-chain running maxima (updated in place; the caller
 // starts them at 0 like src/dqmc_core.cpp:211-212).  Sets HEALTH_NONFINITE_G for a non-finite
 // entry on either side and HEALTH_WATCHDOG when *bound > 0 and the max-abs difference exceeds it.
-// part: per-chain scratch of 16 x 3 doubles for the two-stage reduction.
+// part: per-chain scratch of CMP_PART_SLOTS x 3 doubles for the two-stage reduction.
+#define CMP_PART_SLOTS 16
 hipError_t launch_compare_commit(const double* Gnew, double* G64, float* G32, long long sG, int N,
                                  double* err_signed, double* err_abs, const double* bound, uint32_t* health,
                                  double* part, int batch, hipStream_t s);
+// The second stage alone: folds np partials per chain (signed max, max-abs, non-finite flag) into the
+// running maxima and the health flags, as launch_compare_commit does.
+hipError_t launch_compare_fold(const double* part, int np, double* err_signed, double* err_abs, const double* bound,
+                               uint32_t* health, int batch, hipStream_t s);
 // Same but the working matrix is FP64 only (dynamic G(t,0), G(0,t)); no commit.
 hipError_t launch_compare_only(const double* Gnew, const double* Gcur, long long sG, int N, double* err_signed,
                                uint32_t* health, int batch, hipStream_t s);

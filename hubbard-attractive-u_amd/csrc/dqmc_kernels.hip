@@ -213,7 +213,7 @@ __device__ __forceinline__ double block_max(double v, double* red)
 // commits them and writes its partial (signed max, max-abs, non-finite flag) to part[b][p].
 // Stage 2: one thread per chain folds the parts in order into the running maxima and the health
 // flags (no atomics: the same bits on every run).
-#define CMP_PARTS 16
+#define CMP_PARTS CMP_PART_SLOTS
 __global__ __launch_bounds__(256) void compare_commit_kernel(const double* __restrict__ Gnew, double* __restrict__ G64,
                                                              float* __restrict__ G32, long long sG, int N, int span,
                                                              double* __restrict__ part)
@@ -277,6 +277,14 @@ hipError_t launch_compare_commit(const double* Gnew, double* G64, float* G32, lo
     hipLaunchKernelGGL(compare_commit_kernel, dim3(np, batch), dim3(256), 0, s, Gnew, G64, G32, sG, N, span, part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(compare_fold_kernel, dim3((batch + 63) / 64), dim3(64), 0, s, part, np, err_signed, err_abs,
+                       bound, health, batch);
+    return hipGetLastError();
+}
+
+hipError_t launch_compare_fold(const double* part, int np, double* err_signed, double* err_abs, const double* bound,
+                               uint32_t* health, int batch, hipStream_t s)
+{
     hipLaunchKernelGGL(compare_fold_kernel, dim3((batch + 63) / 64), dim3(64), 0, s, part, np, err_signed, err_abs,
                        bound, health, batch);
     return hipGetLastError();

@@ -10,7 +10,8 @@ against the FP64 CPU oracle on the same seeds and fields.
 * The time-displaced sweep at 8x8, L = 80, beta = 8, mu = -0.5.
 * U = 0 sweeps against the closed form G = (1 + e^{-beta K})^{-1}.
 * The wrap error (signed max, running max from 0) with a planted G, numeric-failure signalling
-  and the watchdog.
+  and the watchdog; at 16x16 also through the general recompose, whose last product compares and
+  commits G in its epilogue.
 * The mixed-precision bound of the FP32 wraps and flushes, recorded as an expected failure.
 
 Decisions (accept masks, fields, RNG state) must be identical.  G must agree within the oracle's
@@ -214,6 +215,35 @@ def test_wrap_error_signed_running_max_planted():
     assert e.stats(1)["wrap_eq"] == 0.0    # the running max starts at 0
     assert e.stats(0)["max_abs_dG"] == 1e-3
     assert np.array_equal(e.gf(0), 0.5 * np.eye(n))
+    e.close()
+
+
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_wrap_error_planted_general_recompose(monkeypatch, fuse):
+    # 16 x 16, U = 0, nnt = 0: B = I and G = I/2 exactly; nt = 8 with pace 4, so the up sweep's first
+    # stabilization has both stacks non-empty (the general recompose, whose last product compares and
+    # commits G in its epilogue unless DQMC_RECOMPOSE_FUSE=0); the 128 x 128 tile is forced
+    say("case: planted wrap error, 16x16 general recompose, DQMC_RECOMPOSE_FUSE=%s" % fuse)
+    monkeypatch.setenv("DQMC_GEMM_TILE", "4")
+    monkeypatch.setenv("DQMC_RECOMPOSE_FUSE", fuse)
+    n = 256
+    e = gpu.Engine(16, 8, 1.0, 0.0, mu=0.0, nnt=0.0, pace=4, n_chains=2, delay_k=4)
+    for c in range(2):
+        e.set_seed(c, 7 + c)
+        e.fields_random(c)
+    e.init()
+    assert np.array_equal(e.gf(0), 0.5 * np.eye(n))
+    p = np.zeros((n, n))
+    p[1, 200] = 1e-3            # recompute - working = -1e-3: the largest |difference|
+    p[250, 3] = -2e-4           # recompute - working = +2e-4: the signed max
+    e.set_gf(0, 0.5 * np.eye(n) + p)
+    e.set_gf(1, 0.5 * np.eye(n) + 1e-3)    # every difference negative
+    e.sweep_up()
+    assert e.stats(0)["wrap_eq"] == 2e-4
+    assert e.stats(1)["wrap_eq"] == 0.0    # the running max starts at 0
+    assert e.stats(0)["max_abs_dG"] == 1e-3
+    assert np.array_equal(e.gf(0), 0.5 * np.eye(n))
+    assert np.array_equal(e.gf(1), 0.5 * np.eye(n))
     e.close()
 
 
