@@ -168,6 +168,9 @@ struct dqmc_ctx {
     double *meas_cache = nullptr, *meas_data = nullptr, *meas_part = nullptr, *costab = nullptr;
     double* wd_bound = nullptr;      // watchdog bound on max|G_recomputed - G_working| (0 = off)
     double* cmp_part = nullptr;      // [B][CMP_PART_SLOTS][3] partial reductions of compare_commit
+    double* colnorm = nullptr;       // [B][N] squared column norms of a pushed block's chain (16 x 16)
+    bool colnorm_ready = false;      // colnorm holds those of the chain in stk
+    bool norm_fuse = true;           // DQMC_NORM_FUSE=0: the pre-pivot reads the chain for its norms
     DStack left, right;
     // parameter adjoint at fixed fields (allocated on first use)
     int* nb = nullptr;
@@ -528,6 +531,7 @@ void chain_begin(dqmc_ctx* c, DStack& s)
     c->chain_stack = &s;
     c->chain_first = true;
     c->chain_npend = 0;
+    c->colnorm_ready = false;
 }
 // One factor of the block's B-chain: stk <- B(t) stk (src/dqmc_core.cpp:190) or B(t)^T stk (:251, adjB,
 // src/hubbard.cpp:333-341).  The first factor of a block reads the top U of the stack the block goes to
@@ -563,6 +567,9 @@ int chain_flush(dqmc_ctx* c, bool last)
     k.adjoint = c->chain_pend_adj ? 1 : 0;
     k.cold = (last && s.len > 0) ? dvec(DIAG_VEC, sD(c, s, s.len - 1), (long long)c->S * c->N) : dnone();
     k.batch = c->B;
+    // the block's last pass also sums its columns' squared norms for the push's pre-pivot
+    k.norms = (last && c->norm_fuse) ? c->colnorm : nullptr;
+    c->colnorm_ready = last && c->norm_fuse;
     TIMED(c, KT_KRON, launch_kron_chain16(k, c->stream));
     std::swap(c->stk, c->stk_alt);
     c->chain_first = false;
@@ -602,7 +609,9 @@ int stack_push(dqmc_ctx* c, DStack& s)
     if (c->chain_stack != &s || c->chain_first) return c->fail(DQMC_ESTATE, "stack push without its B-chain");
     // M = (stacks * U_top) * diag(D_top) (svd_stack.hpp:144), formed by the block's chain_mul calls
     const double* M = c->stk;
-    HCHECK(c, launch_udv_prepare(M, nn, c->qr, c->sqr(), c->weq(), c->perm, N, c->B, c->stream));
+    HCHECK(c, launch_udv_prepare(M, nn, c->qr, c->sqr(), c->weq(), c->perm, N, c->B, c->stream,
+                                 c->colnorm_ready ? c->colnorm : nullptr));
+    c->colnorm_ready = false;
     TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, c->weq(), c->B, c->stream));
     HCHECK(c, launch_udv_finish(c->qr, c->sqr(), c->weq(), c->perm, sU(c, s, s.len), st, sD(c, s, s.len), sd, c->W2, nn, N,
                                 c->health, c->B, c->stream));
@@ -1107,6 +1116,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         }
         const char* w = std::getenv("DQMC_WRAP");
         c->kron = !(w && std::string(w) == "dense");
+        const char* nf = std::getenv("DQMC_NORM_FUSE");
+        c->norm_fuse = !(nf && std::string(nf) == "0");
         const char* rf = std::getenv("DQMC_RECOMPOSE_FUSE");
         c->recompose_fuse = !(rf && std::string(rf) == "0");
         const char* pf = std::getenv("DQMC_PANEL_FUSE");
@@ -1197,6 +1208,7 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
     RCHECK(c->alloc(&c->maxdG, (size_t)B * 8));
     RCHECK(c->alloc(&c->health, (size_t)B * 4)); RCHECK(c->alloc(&c->wd_bound, 8));
     RCHECK(c->alloc(&c->cmp_part, (size_t)B * CMP_PART_SLOTS * 3 * 8));
+    RCHECK(c->alloc(&c->colnorm, (size_t)B * N * 8));
 
     // constant tables
     std::vector<double> ek, iek;

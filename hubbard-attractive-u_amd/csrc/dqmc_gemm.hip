@@ -835,11 +835,22 @@ __global__ __launch_bounds__(256) void kron_chain16_kernel(KronChainArgs a)
             for (int r = 0; r < 4; ++r) v[cc][r] = (z[cc][r] * cs) * dout[r];
         }
     }
+    // the squared norm of a column held at sites lane + 64 s: summed over s in order, then over the
+    // lanes by wave_sum_dpp, as udv_rank_kernel sums a column it reads
+    auto column_norm = [&](int col, const double (&zv)[4]) {
+        double sq = 0.0;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) sq = fma(zv[s], zv[s], sq);
+        sq = wave_sum_dpp(sq);
+        if (lane == 0) a.norms[(long long)b * 256 + col] = sq;
+    };
     if (!tr) {
 #pragma unroll
-        for (int cc = 0; cc < 4; ++cc)
+        for (int cc = 0; cc < 4; ++cc) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) C[(long long)(cbase + cc) * 256 + lane + 64 * r] = v[cc][r];
+            if (a.norms) column_norm(cbase + cc, v[cc]);
+        }
     } else {
         // slot (l, r) holds site (4 r + l/16) + 16 (l%16): back to consecutive sites through LDS
 #pragma unroll
@@ -853,6 +864,7 @@ __global__ __launch_bounds__(256) void kron_chain16_kernel(KronChainArgs a)
             wave_lds_sync();   // read before the next column overwrites the buffer
 #pragma unroll
             for (int s = 0; s < 4; ++s) C[(long long)(cbase + cc) * 256 + lane + 64 * s] = zv[s];
+            if (a.norms) column_norm(cbase + cc, zv);
         }
     }
 }

@@ -79,6 +79,7 @@ struct KronChainArgs {
     int n, adjoint;
     DiagArg cold;
     int batch;
+    double* norms = nullptr;         // optional: the squared 2-norms of C's columns, 256 per chain
 };
 hipError_t launch_kron_chain16(const KronChainArgs& a, hipStream_t s);
 // Host: the message of the calling thread's last failed call without a context
@@ -215,8 +216,10 @@ hipError_t launch_trsm_upper_v1(double* buf, long long stride, int N, int W, int
 
 // Pre-pivoted UDV input: M column-major (ld N); writes buf[r][j] = M(r, perm[j]) with perm
 // sorting the columns of M by descending 2-norm, and buf[r][N+j] = delta_rj (row length W >= 2N).
+// `norms` (N per chain), when given, holds the columns' squared norms as the rank kernel would sum
+// them, so that M is not read for them.
 hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long long sbuf, int W, int* perm,
-                              int N, int batch, hipStream_t s);
+                              int N, int batch, hipStream_t s, const double* norms = nullptr);
 // After QR: D_i = |R_ii|, Tt (column-major N x N) with Tt(perm[j], i) = R_ij / D_i (j >= i),
 // U (column-major) = Q read from the trailing block.
 hipError_t launch_udv_finish(const double* buf, long long sbuf, int W, const int* perm, double* U, long long sU,

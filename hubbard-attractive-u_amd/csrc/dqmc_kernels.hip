@@ -11,16 +11,20 @@ namespace dqmc {
 // ---------------------------------------------------------------------------------------------
 // Column norms of M (column-major, ld N) and the descending-norm order (ties to the lower index,
 // a stable sort): one workgroup per chain, wave w reduces the column groups 8 w .. 8 w + 7,
-// 8 w + 32 .., with its lanes reading consecutive rows.
+// 8 w + 32 .., with its lanes reading consecutive rows (lane l sums rows l, l + 64, .. in order, then
+// wave_sum_dpp).  With norms_in the squared norms are read from there (N per chain) instead.
 __global__ __launch_bounds__(256) void udv_rank_kernel(const double* __restrict__ Ms, long long sM,
-                                                       int* __restrict__ perms, int N)
+                                                       int* __restrict__ perms, int N,
+                                                       const double* __restrict__ norms_in)
 {
     __shared__ double nrm[1024];
     const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const double* M = Ms + (long long)b * sM;
     int* perm = perms + (long long)b * N;
+    if (norms_in)
+        for (int j = tid; j < N; j += 256) nrm[j] = norms_in[(long long)b * N + j];
     // 8 columns per step, so that 8 x ceil(N / 64) loads per lane are in flight together
-    for (int j0 = wave * 8; j0 < N; j0 += 32) {
+    for (int j0 = wave * 8; !norms_in && j0 < N; j0 += 32) {
         double s[8];
 #pragma unroll
         for (int c = 0; c < 8; ++c) s[c] = 0.0;
@@ -29,7 +33,7 @@ __global__ __launch_bounds__(256) void udv_rank_kernel(const double* __restrict_
 #pragma unroll
             for (int c = 0; c < 8; ++c) v[c] = (j0 + c < N) ? M[r + (long long)(j0 + c) * N] : 0.0;
 #pragma unroll
-            for (int c = 0; c < 8; ++c) s[c] += v[c] * v[c];
+            for (int c = 0; c < 8; ++c) s[c] = fma(v[c], v[c], s[c]);
         }
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
@@ -78,10 +82,10 @@ __global__ __launch_bounds__(256) void udv_copy_kernel(const double* __restrict_
 }
 
 hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long long sbuf, int W, int* perm, int N,
-                              int batch, hipStream_t s)
+                              int batch, hipStream_t s, const double* norms)
 {
     if (N > 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(udv_rank_kernel, dim3(batch), dim3(256), 0, s, M, sM, perm, N);
+    hipLaunchKernelGGL(udv_rank_kernel, dim3(batch), dim3(256), 0, s, M, sM, perm, N, norms);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int nt = (N + 31) / 32;
