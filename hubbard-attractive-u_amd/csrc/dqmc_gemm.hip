@@ -375,7 +375,21 @@ __device__ __forceinline__ void gemm128_body(const GemmArgs<double>& g, const Ge
     double es = -DBL_MAX, ea = 0.0;
     bool bad = false;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
+        // the previous values of this row of MFMA tiles (COMMIT, or beta != 0), loaded together before any
+        // of its stores: a load after a store to the same array would otherwise wait for it
+        double prev[4][4];
+        if (COMMIT || g.beta != 0.0) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gi = m0 + wm * WM + i * 16 + Mfma<double>::row(lane, r);
+                    const int gj = n0 + wn * WN + j * 16 + (lane & 15);
+                    const long long off = g.tC ? (gj + (long long)gi * g.ldc) : (gi + (long long)gj * g.ldc);
+                    prev[j][r] = (gi < g.M && gj < g.Nc) ? C[off] : 0.0;
+                }
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -388,16 +402,17 @@ __device__ __forceinline__ void gemm128_body(const GemmArgs<double>& g, const Ge
                     if (!PAIR && g.cold.mode != DIAG_NONE) v *= scol[0][gj - n0];
                     const long long off = g.tC ? (gj + (long long)gi * g.ldc) : (gi + (long long)gj * g.ldc);
                     if (COMMIT) {
-                        const double old = C[off], d = v - old;
+                        const double old = prev[j][r], d = v - old;
                         es = fmax(es, d);
                         ea = fmax(ea, fabs(d));
                         bad |= !isfinite(v) || !isfinite(old);
                     } else if (g.beta != 0.0) {
-                        v += g.beta * C[off];
+                        v += g.beta * prev[j][r];
                     }
                     C[off] = v;
                 }
             }
+    }
     if constexpr (COMMIT) {
         // the tile's maxima over its four waves (max is exact: the order of the pairings is free)
         __shared__ double red[3][4];
