@@ -1082,8 +1082,9 @@ __global__ __launch_bounds__(256) void lu_panel_kernel(double* __restrict__ bufs
 // Trailing update of one LU panel on a 32-column tile, 256 threads: wave w holds the 16-row tiles
 // rt = w, w + 4, w + 8, w + 12 of the tile, rows loaded in the panel's row order (Ps) straight into
 // f64 accumulator layout (item r of tile (rt, ct) = At(16 rt + 4 r + l/16, 16 ct + l%16)).  Waves 0
-// and 1 hold the QRB panel rows A12: through LDS they form U12 = L11^{-1} A12, which every wave then
-// reads as B-fragments for A22 -= L21 U12 on its own row tiles.  The tile is read and written once.
+// and 1 hold the QRB panel rows A12: through LDS the four waves form U12 = L11^{-1} A12 (one 16 x 16
+// tile each), which every wave then reads as B-fragments for A22 -= L21 U12 on its own row tiles.  The
+// tile is read and written once.
 // Four waves per SIMD (at most 128 registers): 145 instead of 164 us for the first panel's update at
 // 256 chains.
 // The update of column tile cb of chain b by one 256-thread workgroup (all threads call it); S is
@@ -1124,19 +1125,24 @@ __device__ __forceinline__ void lu_update_tile(double* __restrict__ bufs, long l
             for (int r = 0; r < 4; ++r) S[0][16 * wave + 4 * r + lr][16 * ct + lc] = at[0][ct][r];
     }
     __syncthreads();
-    if (wave < 2) {
+    {
+        // U12 tile (wave / 2, wave % 2) by each wave
+        const int mt = wave >> 1, ct = wave & 1;
+        acc_t acc = acc_t{0, 0, 0, 0};
 #pragma unroll
-        for (int ct = 0; ct < 2; ++ct) {
-            acc_t acc = acc_t{0, 0, 0, 0};
+        for (int k0 = 0; k0 < QRB; k0 += 4)
+            acc = Mfma<double>::mma(Lv[(16 * mt + lc) + (long long)(k0 + lr) * QRB], S[0][k0 + lr][16 * ct + lc], acc);
 #pragma unroll
-            for (int k0 = 0; k0 < QRB; k0 += 4)
-                acc = Mfma<double>::mma(Lv[(16 * wave + lc) + (long long)(k0 + lr) * QRB], S[0][k0 + lr][16 * ct + lc], acc);
-            at[0][ct] = acc;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) S[1][16 * wave + 4 * r + lr][16 * ct + lc] = acc[r];
-        }
+        for (int r = 0; r < 4; ++r) S[1][16 * mt + 4 * r + lr][16 * ct + lc] = acc[r];
     }
     __syncthreads();
+    if (wave < 2) {
+        // the U12 rows of this wave's first row tile
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) at[0][ct][r] = S[1][16 * wave + 4 * r + lr][16 * ct + lc];
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const int rt = wave + 4 * q;
