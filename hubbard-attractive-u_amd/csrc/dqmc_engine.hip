@@ -171,6 +171,7 @@ struct dqmc_ctx {
     double* colnorm = nullptr;       // [B][N] squared column norms of a pushed block's chain (16 x 16)
     bool colnorm_ready = false;      // colnorm holds those of the chain in stk
     bool norm_fuse = true;           // DQMC_NORM_FUSE=0: the pre-pivot reads the chain for its norms
+    bool rhs_identity = true;        // DQMC_RHS_IDENTITY=0: the pre-pivot writes the push QR's identity
     DStack left, right;
     // parameter adjoint at fixed fields (allocated on first use)
     int* nb = nullptr;
@@ -315,12 +316,19 @@ int gemm(dqmc_ctx* c, const T* A, long long sA, int lda, int tA, const T* B, lon
     return 0;
 }
 
-hipError_t qr_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int batch, hipStream_t s)
+// Whether the push's QR takes its right-hand side [N, 2N) as the identity without reading it (the blocked
+// QR with the fused update at N <= 256), so that the pre-pivot need not write it.
+bool qr_takes_identity(const dqmc_ctx* c)
+{
+    return c->rhs_identity && c->qr_mode != 1 && c->qr_update == QR_UPDATE_REG && c->N <= 256;
+}
+hipError_t qr_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int batch, hipStream_t s,
+                  bool rhs_identity = false)
 {
     if (c->qr_mode == 1) return launch_qr_v1(buf, stride, N, W, batch, s);
     const LookAhead la = c->la();
     return launch_qr_blocked(buf, stride, N, W, batch, c->qrV, c->qrT, c->qrWt, (long long)c->wdyn() * 32, c->tile64, c->qr_update,
-                             c->qr_panel, s, &la);
+                             c->qr_panel, s, &la, rhs_identity);
 }
 hipError_t trsm_any(dqmc_ctx* c, double* buf, long long stride, int N, int W, int c0, int P, int batch, hipStream_t s)
 {
@@ -609,10 +617,11 @@ int stack_push(dqmc_ctx* c, DStack& s)
     if (c->chain_stack != &s || c->chain_first) return c->fail(DQMC_ESTATE, "stack push without its B-chain");
     // M = (stacks * U_top) * diag(D_top) (svd_stack.hpp:144), formed by the block's chain_mul calls
     const double* M = c->stk;
+    const bool ident = qr_takes_identity(c);
     HCHECK(c, launch_udv_prepare(M, nn, c->qr, c->sqr(), c->weq(), c->perm, N, c->B, c->stream,
-                                 c->colnorm_ready ? c->colnorm : nullptr));
+                                 c->colnorm_ready ? c->colnorm : nullptr, !ident));
     c->colnorm_ready = false;
-    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, c->weq(), c->B, c->stream));
+    TIMED(c, KT_QR, qr_any(c, c->qr, c->sqr(), N, c->weq(), c->B, c->stream, ident));
     HCHECK(c, launch_udv_finish(c->qr, c->sqr(), c->weq(), c->perm, sU(c, s, s.len), st, sD(c, s, s.len), sd, c->W2, nn, N,
                                 c->health, c->B, c->stream));
     if (s.len == 0)
@@ -1116,6 +1125,8 @@ int dqmc_create(const dqmc_params_c* p, int n_chains, int precision, int delay_k
         }
         const char* w = std::getenv("DQMC_WRAP");
         c->kron = !(w && std::string(w) == "dense");
+        const char* ri = std::getenv("DQMC_RHS_IDENTITY");
+        c->rhs_identity = !(ri && std::string(ri) == "0");
         const char* nf = std::getenv("DQMC_NORM_FUSE");
         c->norm_fuse = !(nf && std::string(nf) == "0");
         const char* rf = std::getenv("DQMC_RECOMPOSE_FUSE");

@@ -196,9 +196,10 @@ struct LookAhead {
     bool fuse = true;   // panel k + 1 applies update k to its own columns (no separate block-0 launch)
 };
 enum QrPanel : int { QR_PANEL_256 = 0, QR_PANEL_1024 = 1 };
+// rhs_identity: columns [N, 2N) are the identity and are not read (QR_UPDATE_REG, N <= 256, W >= 2N).
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
                              double* Wt, long long wt_stride, int tile, int update, int panel, hipStream_t s,
-                             const LookAhead* la = nullptr);
+                             const LookAhead* la = nullptr, bool rhs_identity = false);
 // Blocked LU with partial pivoting of the leading N x N block of the same row-major N x W system
 // (N <= 256), applied to the W - N trailing columns: on exit the leading block holds U and the
 // trailing columns L^{-1} P rhs.  Scratch per chain: Ls N*32 doubles, Li 32*32 doubles, Ps N ints.
@@ -217,9 +218,10 @@ hipError_t launch_trsm_upper_v1(double* buf, long long stride, int N, int W, int
 // Pre-pivoted UDV input: M column-major (ld N); writes buf[r][j] = M(r, perm[j]) with perm
 // sorting the columns of M by descending 2-norm, and buf[r][N+j] = delta_rj (row length W >= 2N).
 // `norms` (N per chain), when given, holds the columns' squared norms as the rank kernel would sum
-// them, so that M is not read for them.
+// them, so that M is not read for them.  Without `identity` the delta columns are left unwritten (for
+// a QR that takes them as the identity).
 hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long long sbuf, int W, int* perm,
-                              int N, int batch, hipStream_t s, const double* norms = nullptr);
+                              int N, int batch, hipStream_t s, const double* norms = nullptr, bool identity = true);
 // After QR: D_i = |R_ii|, Tt (column-major N x N) with Tt(perm[j], i) = R_ij / D_i (j >= i),
 // U (column-major) = Q read from the trailing block.
 hipError_t launch_udv_finish(const double* buf, long long sbuf, int W, const int* perm, double* U, long long sU,

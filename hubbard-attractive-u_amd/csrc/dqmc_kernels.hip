@@ -53,11 +53,11 @@ __global__ __launch_bounds__(256) void udv_rank_kernel(const double* __restrict_
     }
 }
 
-// buf[r][j] = M(r, perm[j]) and buf[r][N + j] = delta_rj, through a 32 x 32 LDS tile so that the
+// buf[r][j] = M(r, perm[j]) and, with `identity`, buf[r][N + j] = delta_rj, through a 32 x 32 LDS tile so that the
 // column reads of M and the row writes of buf are both contiguous; grid (tiles, chains).
 __global__ __launch_bounds__(256) void udv_copy_kernel(const double* __restrict__ Ms, long long sM,
                                                        double* __restrict__ bufs, long long sbuf,
-                                                       const int* __restrict__ perms, int N, int W)
+                                                       const int* __restrict__ perms, int N, int W, int identity)
 {
     __shared__ double t[32][33];
     const int b = blockIdx.y, nt = (N + 31) / 32;
@@ -76,20 +76,21 @@ __global__ __launch_bounds__(256) void udv_copy_kernel(const double* __restrict_
         const int r = r0 + rr, j = j0 + jj;
         if (r < N && j < N) {
             buf[(long long)r * W + j] = t[jj][rr];
-            buf[(long long)r * W + N + j] = (r == j) ? 1.0 : 0.0;
+            if (identity) buf[(long long)r * W + N + j] = (r == j) ? 1.0 : 0.0;
         }
     }
 }
 
 hipError_t launch_udv_prepare(const double* M, long long sM, double* buf, long long sbuf, int W, int* perm, int N,
-                              int batch, hipStream_t s, const double* norms)
+                              int batch, hipStream_t s, const double* norms, bool identity)
 {
     if (N > 1024) return hipErrorInvalidValue;
     hipLaunchKernelGGL(udv_rank_kernel, dim3(batch), dim3(256), 0, s, M, sM, perm, N, norms);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int nt = (N + 31) / 32;
-    hipLaunchKernelGGL(udv_copy_kernel, dim3(nt * nt, batch), dim3(256), 0, s, M, sM, buf, sbuf, perm, N, W);
+    hipLaunchKernelGGL(udv_copy_kernel, dim3(nt * nt, batch), dim3(256), 0, s, M, sM, buf, sbuf, perm, N, W,
+                       identity ? 1 : 0);
     return hipGetLastError();
 }
 

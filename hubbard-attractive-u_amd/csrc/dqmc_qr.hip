@@ -390,15 +390,18 @@ __device__ __forceinline__ void panel_rows_store(double (*tp)[QRB / 2 + 1], cons
 __device__ __forceinline__ void qr_update_tile(double* __restrict__ bufs, long long stride, int N, int W, int b, int cb,
                                                int p0, int nb, const double* __restrict__ Vs,
                                                const double* __restrict__ Vrows, long long vr_stride,
-                                               const double* __restrict__ Ts, double (*Wp)[QRB][QRB + 1]);
+                                               const double* __restrict__ Ts, double (*Wp)[QRB][QRB + 1],
+                                               int ident = 0x7fffffff);
 // With pp0 >= 0 the workgroup first applies the previous panel's reflectors (V, T of the panel at row
 // pp0, row-major copy pVr) to this panel's columns, as the trailing update's tile 0 would, and then
-// factors them: one launch, and one kernel boundary, fewer per panel.
+// factors them: one launch, and one kernel boundary, fewer per panel.  `pident` is that update's
+// identity column (qr_update_tile).
 __global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs, long long stride, int N, int W,
                                                        int p0, int nb, double* __restrict__ Vs,
                                                        double* __restrict__ Ts, double* __restrict__ Vrows,
                                                        long long vr_stride, int pp0, const double* __restrict__ pV,
-                                                       const double* __restrict__ pT, const double* __restrict__ pVr)
+                                                       const double* __restrict__ pT, const double* __restrict__ pVr,
+                                                       int pident)
 {
     // latency-bound: issue ahead of the update kernels that share the CU
     __builtin_amdgcn_s_setprio(3);
@@ -420,7 +423,7 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs
         // tp is free until the write-back; this workgroup's stores of the tile are visible to it past
         // the barrier
         qr_update_tile(bufs, stride, N, W, b, 0, pp0, (N - pp0 < QRB) ? N - pp0 : QRB, pV, pVr, vr_stride, pT,
-                       reinterpret_cast<double(*)[QRB][QRB + 1]>(&tp[0][0]));
+                       reinterpret_cast<double(*)[QRB][QRB + 1]>(&tp[0][0]), pident);
         __syncthreads();
     }
 
@@ -512,11 +515,14 @@ __global__ __launch_bounds__(256) void qr_panel_kernel(double* __restrict__ bufs
 //      B-fragments of W2^T from LDS), and stores them.
 // At is read and written exactly once; V and T come from L2.
 // The update of column tile cb of chain b by one 256-thread workgroup (all threads call it); Wp is
-// 4 x QRB x (QRB + 1) doubles of LDS.
+// 4 x QRB x (QRB + 1) doubles of LDS.  Columns from `ident` on are taken as those of the identity,
+// column ident + j being e_j (the untouched right-hand side of the first panel's update), instead of
+// being read.
 __device__ __forceinline__ void qr_update_tile(double* __restrict__ bufs, long long stride, int N, int W, int b, int cb,
                                                int p0, int nb, const double* __restrict__ Vs,
                                                const double* __restrict__ Vrows, long long vr_stride,
-                                               const double* __restrict__ Ts, double (*Wp)[QRB][QRB + 1])
+                                               const double* __restrict__ Ts, double (*Wp)[QRB][QRB + 1],
+                                               int ident)
 {
     typedef Mfma<double>::acc_t acc_t;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -537,7 +543,9 @@ __device__ __forceinline__ void qr_update_tile(double* __restrict__ bufs, long l
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int row = 16 * (wave + 4 * q) + 4 * r + lr, col = 16 * ct + lc;
-                at[q][ct][r] = (row < R && c0 + col < cols) ? A[base + (long long)row * W + col] : 0.0;
+                const int a = p0 + nb + c0 + col;   // the column in the system
+                at[q][ct][r] = !(row < R && c0 + col < cols) ? 0.0
+                               : (a >= ident ? (p0 + row == a - ident ? 1.0 : 0.0) : A[base + (long long)row * W + col]);
             }
 
     // 1. partial W(i, m) = sum over this wave's rows k of At(k, i) V(k, m)
@@ -627,12 +635,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
                                                             int W, int batch, int p0, int nb, int cb0, int ncb,
                                                             const double* __restrict__ Vs,
                                                             const double* __restrict__ Vrows, long long vr_stride,
-                                                            const double* __restrict__ Ts)
+                                                            const double* __restrict__ Ts, int ident)
 {
     __shared__ double Wp[4][QRB][QRB + 1];
     int b, cb;
     chain_tile(ncb, batch, b, cb);
-    qr_update_tile(bufs, stride, N, W, b, cb + cb0, p0, nb, Vs, Vrows, vr_stride, Ts, Wp);
+    qr_update_tile(bufs, stride, N, W, b, cb + cb0, p0, nb, Vs, Vrows, vr_stride, Ts, Wp, ident);
 }
 
 
@@ -760,8 +768,11 @@ __global__ __launch_bounds__(64 * NW) void qr_panel_wide_kernel(double* __restri
 
 hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int batch, double* Vs, double* Ts,
                              double* Wt, long long wt_stride, int tile, int update, int panel, hipStream_t s,
-                             const LookAhead* la)
+                             const LookAhead* la, bool rhs_identity)
 {
+    // the first panel's update reads the columns from N on as the identity (QR_UPDATE_REG only)
+    if (rhs_identity && (update != QR_UPDATE_REG || N > 256 || W < 2 * N)) return hipErrorInvalidValue;
+    const int ident0 = rhs_identity ? N : 0x7fffffff;
     // per chain: Vs N*QRB, Ts QRB*QRB, Wt 2 * wt_stride (Wt and Wt T; in QR_UPDATE_REG the first
     // holds the row-major copy of V, the second that of the look-ahead's parity-1 V), wt_stride >= W*QRB
     if (N > QR_WIDE_MAXN || N < 1 || W <= N) return hipErrorInvalidValue;
@@ -780,14 +791,14 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
                                wt_stride);
         else
             hipLaunchKernelGGL(qr_panel_kernel, dim3(batch), dim3(256), 0, st, buf, stride, N, W, p0, nb, V, T, Vr,
-                               wt_stride, pp0, pV, pT, pVr);
+                               wt_stride, pp0, pV, pT, pVr, pp0 == 0 ? ident0 : 0x7fffffff);
         return hipGetLastError();
     };
     auto launch_update = [&](int p0, int par, int cb0, int ncb, hipStream_t st) {
         const int nb = (N - p0 < QRB) ? N - p0 : QRB;
         double *V = par ? la->V2 : Vs, *T = par ? la->T2 : Ts, *Vr = par ? Wt2 : Wt;
         hipLaunchKernelGGL(qr_update_kernel, dim3(ncb * batch), dim3(256), 0, st, buf, stride, N, W, batch, p0, nb,
-                           cb0, ncb, V, Vr, wt_stride, T);
+                           cb0, ncb, V, Vr, wt_stride, T, p0 == 0 ? ident0 : 0x7fffffff);
         return hipGetLastError();
     };
     if (la && la->s2 && update == QR_UPDATE_REG) {
@@ -838,13 +849,14 @@ hipError_t launch_qr_blocked(double* buf, long long stride, int N, int W, int ba
         else
             hipLaunchKernelGGL(qr_panel_kernel, dim3(batch), dim3(256), 0, s, buf, stride, N, W, p0, nb, Vs, Ts,
                                reg ? Wt : nullptr, wt_stride, -1, (const double*)nullptr, (const double*)nullptr,
-                               (const double*)nullptr);
+                               (const double*)nullptr, 0x7fffffff);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         if (cols <= 0) continue;
         if (reg) {
             hipLaunchKernelGGL(qr_update_kernel, dim3(((cols + QRB - 1) / QRB) * batch), dim3(256), 0, s, buf,
-                               stride, N, W, batch, p0, nb, 0, (cols + QRB - 1) / QRB, Vs, Wt, wt_stride, Ts);
+                               stride, N, W, batch, p0, nb, 0, (cols + QRB - 1) / QRB, Vs, Wt, wt_stride, Ts,
+                               p0 == 0 ? ident0 : 0x7fffffff);
             e = hipGetLastError();
             if (e != hipSuccess) return e;
             continue;

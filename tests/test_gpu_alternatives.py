@@ -22,6 +22,8 @@ one exercised).
                          stabilization's compare-and-commit as its own pass
   DQMC_NORM_FUSE=0       the push's pre-pivot reading the B-chain for its column norms, instead of
                          taking them from the B-chain kernel's last pass
+  DQMC_RHS_IDENTITY=0    the push's pre-pivot writing the identity right-hand side of its QR, instead of
+                         the QR's first update taking those columns as the identity unread
   DQMC_QR=v1             one-workgroup unblocked Householder QR instead of the blocked one
   DQMC_METRO_THREADS     Metropolis workgroup of 256 or 512 threads instead of 1024
   DQMC_LOOKAHEAD=0       blocked QR and LU panels and updates in order on one stream, instead of
@@ -84,7 +86,8 @@ def default_run():
                                      ("DQMC_CHAIN", "slice"), ("DQMC_WRAP_FUSE", "0"), ("DQMC_METRO_SPLIT", "1"),
                                      ("DQMC_METRO_SPLIT", "4"), ("DQMC_PANEL_FUSE", "0"),
                                      ("DQMC_TRSM_BLOCK", "32"), ("DQMC_TRSM_BLOCK", "128"),
-                                     ("DQMC_RECOMPOSE_FUSE", "0"), ("DQMC_NORM_FUSE", "0")])
+                                     ("DQMC_RECOMPOSE_FUSE", "0"), ("DQMC_NORM_FUSE", "0"),
+                                     ("DQMC_RHS_IDENTITY", "0")])
 def test_switch_follows_default_chains(default_run, monkeypatch, var, val):
     say("case: %s=%s, 16x16 L=40, 8 chains, 2 sweeps" % (var, val))
     monkeypatch.setenv(var, val)
