@@ -176,25 +176,42 @@ hipError_t launch_build_rhs(const double* X, long long sX, DiagArg d, double alp
     return hipGetLastError();
 }
 
+// out = the N x N block at column c0 of the row-major system, column-major (transpose = 0: out(j, r) =
+// buf[r][c0 + j]) or as is (transpose = 1: out(r, j) = buf[r][c0 + j]); 32 x 32 tiles through LDS so that
+// the reads and the writes are both contiguous; grid (tiles, chains).
 __global__ __launch_bounds__(256) void extract_kernel(const double* __restrict__ bufs, long long sbuf, int W,
                                                       int c0, double* __restrict__ outs, long long so, int N,
                                                       int transpose)
 {
-    const int b = blockIdx.y;
-    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (long long)N * N) return;
-    const int r = (int)(e / N), j = (int)(e % N);
-    const double v = bufs[(long long)b * sbuf + (long long)r * W + c0 + j];
+    __shared__ double t[32][33];
+    const int b = blockIdx.y, nt = (N + 31) / 32;
+    const int r0 = (blockIdx.x % nt) * 32, j0 = (blockIdx.x / nt) * 32;
+    const double* buf = bufs + (long long)b * sbuf;
     double* out = outs + (long long)b * so;
-    if (transpose) out[r + (long long)j * N] = v;
-    else out[j + (long long)r * N] = v;
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        const int rr = e >> 5, jj = e & 31;   // consecutive threads read consecutive columns of a row
+        const int r = r0 + rr, j = j0 + jj;
+        const double v = (r < N && j < N) ? buf[(long long)r * W + c0 + j] : 0.0;
+        if (!transpose) {
+            if (r < N && j < N) out[j + (long long)r * N] = v;   // already contiguous in j
+        } else {
+            t[jj][rr] = v;
+        }
+    }
+    if (!transpose) return;   // uniform
+    __syncthreads();
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        const int jj = e >> 5, rr = e & 31;   // consecutive threads write consecutive rows of a column
+        const int r = r0 + rr, j = j0 + jj;
+        if (r < N && j < N) out[r + (long long)j * N] = t[jj][rr];
+    }
 }
 
 hipError_t launch_extract(const double* buf, long long sbuf, int W, int c0, double* out, long long so, int N,
                           int transpose, int batch, hipStream_t s)
 {
-    dim3 grid((unsigned)(((long long)N * N + 255) / 256), batch);
-    hipLaunchKernelGGL(extract_kernel, grid, dim3(256), 0, s, buf, sbuf, W, c0, out, so, N, transpose);
+    const int nt = (N + 31) / 32;
+    hipLaunchKernelGGL(extract_kernel, dim3(nt * nt, batch), dim3(256), 0, s, buf, sbuf, W, c0, out, so, N, transpose);
     return hipGetLastError();
 }
 
