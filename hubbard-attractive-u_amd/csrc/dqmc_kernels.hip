@@ -100,36 +100,31 @@ __global__ __launch_bounds__(256) void udv_finish_kernel(const double* __restric
                                                          double* __restrict__ Tts, long long sTt, int N, int W,
                                                          uint32_t* health)
 {
-    __shared__ int iperm[1024];   // iperm[perm[j]] = j
     const int b = blockIdx.y;
     const double* buf = bufs + (long long)b * sbuf;
     const int* perm = perms + (long long)b * N;
     double* U = Us + (long long)b * sU;
     double* D = Ds + (long long)b * sD;
     double* Tt = Tts + (long long)b * sTt;
-    for (int r = threadIdx.x; r < N; r += 256) iperm[perm[r]] = r;
-    __syncthreads();
     const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
     if (e >= (long long)N * N) return;
-    const int i = (int)(e / N), q = (int)(e % N);
-    // U column-major: U(q, i) = Q(q, i) = (Q^T)(i, q) = buf[i][N + q]
-    U[q + (long long)i * N] = buf[(long long)i * W + N + q];
+    const int i = (int)(e / N), j = (int)(e % N);
+    // U column-major: U(j, i) = Q(j, i) = (Q^T)(i, j) = buf[i][N + j]
+    U[j + (long long)i * N] = buf[(long long)i * W + N + j];
     const double d = fabs(buf[(long long)i * W + i]);
-    if (q == 0) {
+    if (j == 0) {
         D[i] = d;
         // a zero or non-finite |R_ii| is a breakdown of the factorization
         if (!(d > 0.0) || !isfinite(d)) atomicOr(health + b, HEALTH_ZERO_PIVOT);
     }
-    // Tt(perm[j], i) = R(i, j) / D_i for j >= i, written at consecutive rows q = perm[j] (R gathered)
-    const int j = iperm[q];
-    Tt[q + (long long)i * N] = (j >= i) ? buf[(long long)i * W + j] / d : 0.0;
+    // Tt(perm[j], i) = R(i, j) / D_i for j >= i
+    Tt[perm[j] + (long long)i * N] = (j >= i) ? buf[(long long)i * W + j] / d : 0.0;
 }
 
 hipError_t launch_udv_finish(const double* buf, long long sbuf, int W, const int* perm, double* U, long long sU,
                              double* D, long long sD, double* Tt, long long sTt, int N, uint32_t* health, int batch,
                              hipStream_t s)
 {
-    if (N > 1024) return hipErrorInvalidValue;
     dim3 grid((unsigned)(((long long)N * N + 255) / 256), batch);
     hipLaunchKernelGGL(udv_finish_kernel, grid, dim3(256), 0, s, buf, sbuf, perm, U, sU, D, sD, Tt, sTt, N, W, health);
     return hipGetLastError();
